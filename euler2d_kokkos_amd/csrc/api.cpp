@@ -1,0 +1,642 @@
+// api.cpp -- C ABI (include/euler2d_hip.h) over the HIP kernels.
+//
+// The handle plays the role of the reference's HydroRun<device_t>
+// (src/HydroRun.h:44-134): it owns U, U2 (and, for the pass schedule, Q,
+// Fluxes_x, Fluxes_y) in device memory as four SoA planes each, one HIP stream,
+// and the device-resident loop control block.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "config.hpp"
+#include "euler2d_hip.h"
+#include "kernels.hpp"
+#include "params.hpp"
+
+using e2d::Grid;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(E2D_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                               \
+  do {                                              \
+    hipError_t e_ = (expr);                         \
+    if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+  } while (0)
+
+// steps launched between two host checks of the loop state
+constexpr int kChunk = 256;
+
+}  // namespace
+
+struct e2d_config {
+  e2d::ConfigMap map;
+};
+
+struct e2d_run {
+  e2d_params p{};
+  e2d::KParams kp{};
+  int precision = 0;
+  int schedule = E2D_SCHEDULE_FUSED;
+  int seg_rows = 16;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Grid U{}, U2{}, Q{}, Fx{}, Fy{};
+  e2d::StepCtl *ctl = nullptr;
+  unsigned long long *bits = nullptr;
+  unsigned int *count = nullptr;
+  double *partials = nullptr;
+  double *sums4 = nullptr;
+  double timers[5] = {0, 0, 0, 0, 0};
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+int alloc_dev(e2d_run *r, void **ptr, size_t bytes) {
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(E2D_ERR_ALLOC, std::string("hipMalloc(") + std::to_string(bytes) +
+                                   " bytes): " + hipGetErrorString(e));
+  }
+  r->allocs.push_back(*ptr);
+  return E2D_OK;
+}
+
+int alloc_grid(e2d_run *r, Grid &g) {
+  const int pitch = (r->p.isize + 15) / 16 * 16;  // rows start on 128-byte boundaries
+  long long plane = (long long)pitch * r->p.jsize;
+  plane = (plane + 63) / 64 * 64;  // planes start on 512-byte boundaries
+  void *ptr = nullptr;
+  int rc = alloc_dev(r, &ptr, (size_t)plane * 4 * sizeof(double));
+  if (rc) return rc;
+  g.p = static_cast<double *>(ptr);
+  g.pitch = pitch;
+  g.plane = plane;
+  return E2D_OK;
+}
+
+Grid &state(e2d_run *r, int which) { return which == E2D_U ? r->U : r->U2; }
+
+int check_run(e2d_run *r) {
+  if (!r) return fail(E2D_ERR_ARG, "null e2d_run handle");
+  hipError_t e = hipSetDevice(r->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return E2D_OK;
+}
+
+int ensure_pass_arrays(e2d_run *r) {
+  if (r->Q.p) return E2D_OK;
+  int rc = alloc_grid(r, r->Q);
+  if (!rc) rc = alloc_grid(r, r->Fx);
+  if (!rc) rc = alloc_grid(r, r->Fy);
+  if (!rc) {
+    const size_t bytes = (size_t)r->Fx.plane * 4 * sizeof(double);
+    HIP_TRY(hipMemsetAsync(r->Fx.p, 0, bytes, r->stream));
+    HIP_TRY(hipMemsetAsync(r->Fy.p, 0, bytes, r->stream));
+  }
+  return rc;
+}
+
+// one godunov step with an explicit dt; in/out by the parity of nStep
+int do_step(e2d_run *r, int nStep, double dt, bool timed) {
+  Grid in = (nStep % 2 == 0) ? r->U : r->U2;
+  Grid out = (nStep % 2 == 0) ? r->U2 : r->U;
+  const double dtdx = dt / r->p.dx;
+  const double dtdy = dt / r->p.dy;
+  hipEvent_t ev[5] = {};
+  if (timed)
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  if (r->schedule == E2D_SCHEDULE_PASSES) {
+    int rc = ensure_pass_arrays(r);
+    if (rc) return rc;
+    if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
+    HIP_TRY(e2d::launch_boundaries(r->kp, in, r->stream));
+    HIP_TRY(hipMemcpyAsync(out.p, in.p, (size_t)in.plane * 4 * sizeof(double),
+                           hipMemcpyDeviceToDevice, r->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[1], r->stream));
+    HIP_TRY(e2d::launch_prim(r->kp, in, r->Q, r->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[2], r->stream));
+    HIP_TRY(e2d::launch_fluxes(r->kp, r->Q, r->Fx, r->Fy, dtdx, dtdy, r->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[3], r->stream));
+    HIP_TRY(e2d::launch_update(r->kp, out, r->Fx, r->Fy, r->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[4], r->stream));
+  } else {
+    if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
+    HIP_TRY(e2d::launch_step_fused(r->kp, in, out, dt, nullptr, r->seg_rows, r->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[4], r->stream));
+  }
+  if (timed) {
+    HIP_TRY(hipEventSynchronize(ev[4]));
+    float ms = 0.f;
+    if (r->schedule == E2D_SCHEDULE_PASSES) {
+      HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      r->timers[0] += ms * 1e-3;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2]));
+      r->timers[2] += ms * 1e-3;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3]));
+      r->timers[3] += ms * 1e-3;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[3], ev[4]));
+      r->timers[4] += ms * 1e-3;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[4]));
+      r->timers[1] += ms * 1e-3;
+    } else {
+      HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[4]));
+      r->timers[1] += ms * 1e-3;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  }
+  return E2D_OK;
+}
+
+// Host-side loop state -> control block; dt of the pending step computed on
+// the device from the current state.
+int loop_begin(e2d_run *r, double t, int nStep) {
+  e2d::StepCtl c{};
+  c.t = t;
+  c.nstep = nStep;
+  HIP_TRY(hipMemcpyAsync(r->ctl, &c, sizeof(c), hipMemcpyHostToDevice, r->stream));
+  HIP_TRY(e2d::launch_dt_reduce(r->kp, (nStep % 2 == 0) ? r->U : r->U2, &r->ctl->invdt,
+                                r->stream));
+  HIP_TRY(e2d::launch_dt_finalize(r->kp, r->ctl, r->stream));
+  return E2D_OK;
+}
+
+int loop_steps(e2d_run *r, int nStep0, int count) {
+  for (int k = 0; k < count; ++k) {
+    const int n = nStep0 + k;
+    Grid in = (n % 2 == 0) ? r->U : r->U2;
+    Grid out = (n % 2 == 0) ? r->U2 : r->U;
+    HIP_TRY(e2d::launch_step_fused(r->kp, in, out, 0.0, r->ctl, r->seg_rows, r->stream));
+  }
+  return E2D_OK;
+}
+
+int loop_read(e2d_run *r, e2d::StepCtl &c) {
+  HIP_TRY(hipMemcpyAsync(&c, r->ctl, sizeof(c), hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return E2D_OK;
+}
+
+int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elapsed_ms,
+                  int *executed) {
+  if (!t || !nStep) return fail(E2D_ERR_ARG, "null t / nStep");
+  if (r->schedule == E2D_SCHEDULE_PASSES) {
+    // reference structure: host dt per step, exactly main.cpp:100-145
+    const int n0 = *nStep;
+    if (*nStep == 0) {
+      HIP_TRY(e2d::launch_boundaries(r->kp, r->U, r->stream));
+      HIP_TRY(e2d::launch_boundaries(r->kp, r->U2, r->stream));
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, r->stream));
+    int done = 0;
+    while (*t < r->p.tEnd && *nStep < r->p.nStepmax && (max_steps <= 0 || done < max_steps)) {
+      double dt;
+      int rc = e2d_compute_dt(r, *nStep % 2, &dt);
+      if (rc) return rc;
+      if (*t + dt > r->p.tEnd) dt = r->p.tEnd - *t;
+      rc = do_step(r, *nStep, dt, false);
+      if (rc) return rc;
+      *nStep += 1;
+      *t += dt;
+      done++;
+    }
+    HIP_TRY(hipEventRecord(e1, r->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    r->timers[1] += ms * 1e-3;
+    if (elapsed_ms) *elapsed_ms = ms;
+    if (executed) *executed = *nStep - n0;
+    return E2D_OK;
+  }
+
+  if (*nStep == 0) {
+    // main.cpp:90-91 (only the ghost cells are affected)
+    HIP_TRY(e2d::launch_boundaries(r->kp, r->U, r->stream));
+    HIP_TRY(e2d::launch_boundaries(r->kp, r->U2, r->stream));
+  }
+  const int n0 = *nStep;
+  int rc = loop_begin(r, *t, *nStep);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, r->stream));
+  e2d::StepCtl c{};
+  int launched = 0;
+  for (;;) {
+    int todo = kChunk;
+    if (max_steps > 0) todo = std::min(todo, max_steps - launched);
+    const long long left = (long long)r->p.nStepmax - (n0 + launched);
+    if (left < todo) todo = (int)std::max(0LL, left);
+    if (todo <= 0) break;
+    rc = loop_steps(r, n0 + launched, todo);
+    if (rc) return rc;
+    launched += todo;
+    rc = loop_read(r, c);
+    if (rc) return rc;
+    if (!c.active) break;
+  }
+  HIP_TRY(hipEventRecord(e1, r->stream));
+  rc = loop_read(r, c);
+  if (rc) return rc;
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  r->timers[1] += ms * 1e-3;
+  *t = c.t;
+  *nStep = c.nstep;
+  if (elapsed_ms) *elapsed_ms = ms;
+  if (executed) *executed = c.nstep - n0;
+  return E2D_OK;
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    return fail(E2D_ERR_ALLOC, "host allocation failed");
+  } catch (...) {
+    return fail(E2D_ERR_ARG, "unexpected C++ exception");
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *e2d_last_error(void) { return g_last_error.c_str(); }
+
+int e2d_version(void) { return 1; }
+
+int e2d_params_size(void) { return (int)sizeof(e2d_params); }
+
+int e2d_device_info(char *name, int name_len, int *num_cu) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(E2D_ERR_HIP, "no HIP device visible");
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, 0);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (num_cu) *num_cu = prop.multiProcessorCount;
+  return E2D_OK;
+}
+
+// ------------------------------------------------------------ ConfigMap
+int e2d_config_open(const char *path, e2d_config **out, int *parse_error) {
+  return guarded([&]() -> int {
+    if (!path || !out) return fail(E2D_ERR_ARG, "null path / out");
+    auto *c = new e2d_config{e2d::ConfigMap(path)};
+    *out = c;
+    if (parse_error) *parse_error = c->map.ParseError();
+    if (c->map.ParseError() == -1) return fail(E2D_ERR_IO, std::string("cannot open ") + path);
+    return E2D_OK;
+  });
+}
+
+int e2d_config_close(e2d_config *cfg) {
+  delete cfg;
+  return E2D_OK;
+}
+
+int e2d_config_get_string(const e2d_config *cfg, const char *section, const char *name,
+                          const char *default_value, char *out, int out_len) {
+  if (!cfg || !section || !name || !default_value) return fail(E2D_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    const std::string v = cfg->map.getString(section, name, default_value);
+    if (out && out_len > 0) {
+      std::strncpy(out, v.c_str(), (size_t)out_len);
+      out[out_len - 1] = '\0';
+    }
+    return (int)v.size();
+  });
+}
+
+int e2d_config_get_integer(const e2d_config *cfg, const char *section, const char *name,
+                           long default_value, long *out) {
+  if (!cfg || !section || !name || !out) return fail(E2D_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    *out = cfg->map.getInteger(section, name, default_value);
+    return E2D_OK;
+  });
+}
+
+int e2d_config_get_float(const e2d_config *cfg, const char *section, const char *name,
+                         float default_value, float *out) {
+  if (!cfg || !section || !name || !out) return fail(E2D_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    *out = cfg->map.getFloat(section, name, default_value);
+    return E2D_OK;
+  });
+}
+
+int e2d_config_get_bool(const e2d_config *cfg, const char *section, const char *name,
+                        int default_value, int *out) {
+  if (!cfg || !section || !name || !out) return fail(E2D_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    *out = cfg->map.getBool(section, name, default_value != 0) ? 1 : 0;
+    return E2D_OK;
+  });
+}
+
+// ---------------------------------------------------------- HydroParams
+int e2d_params_setup(const e2d_config *cfg, e2d_params *out) {
+  if (!cfg || !out) return fail(E2D_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    e2d::params_setup(cfg->map, *out);
+    return E2D_OK;
+  });
+}
+
+int e2d_params_from_ini(const char *path, e2d_params *out, int *parse_error) {
+  if (!path || !out) return fail(E2D_ERR_ARG, "null argument");
+  return guarded([&]() -> int {
+    e2d::ConfigMap map(path);
+    if (parse_error) *parse_error = map.ParseError();
+    e2d::params_setup(map, *out);  // defaults when the file is missing, like the reference
+    if (map.ParseError() == -1) return fail(E2D_ERR_IO, std::string("cannot open ") + path);
+    return E2D_OK;
+  });
+}
+
+int e2d_params_init(e2d_params *p) {
+  if (!p) return fail(E2D_ERR_ARG, "null params");
+  e2d::params_init(*p);
+  return E2D_OK;
+}
+
+int e2d_params_print(const e2d_params *p) {
+  if (!p) return fail(E2D_ERR_ARG, "null params");
+  e2d::params_print(*p);
+  return E2D_OK;
+}
+
+int e2d_params_validate(const e2d_params *p) {
+  if (!p) return fail(E2D_ERR_ARG, "null params");
+  const char *why = e2d::params_check(*p);
+  if (why) return fail(E2D_ERR_UNSUPPORTED, why);
+  return E2D_OK;
+}
+
+// --------------------------------------------------------------- HydroRun
+int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
+  if (!p || !out) return fail(E2D_ERR_ARG, "null argument");
+  *out = nullptr;
+  int rc = e2d_params_validate(p);
+  if (rc) return rc;
+  if (precision != E2D_PRECISION_FP64)
+    return fail(E2D_ERR_UNSUPPORTED, "precision not available in this build");
+  return guarded([&]() -> int {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(E2D_ERR_HIP, "no HIP device visible");
+    auto *r = new e2d_run();
+    r->p = *p;
+    r->kp = e2d::make_kparams(*p);
+    r->precision = precision;
+    HIP_TRY(hipGetDevice(&r->device));
+    e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete r;
+      return hip_fail(e, "hipStreamCreate");
+    }
+    void *ptr = nullptr;
+    int rc2 = alloc_grid(r, r->U);
+    if (!rc2) rc2 = alloc_grid(r, r->U2);
+    if (!rc2) rc2 = alloc_dev(r, &ptr, sizeof(e2d::StepCtl));
+    if (!rc2) r->ctl = static_cast<e2d::StepCtl *>(ptr);
+    if (!rc2) rc2 = alloc_dev(r, &ptr, 64);
+    if (!rc2) r->bits = static_cast<unsigned long long *>(ptr);
+    if (!rc2) rc2 = alloc_dev(r, &ptr, 64);
+    if (!rc2) r->count = static_cast<unsigned int *>(ptr);
+    if (!rc2) rc2 = alloc_dev(r, &ptr, (size_t)e2d::sums_partials_needed(r->kp) * sizeof(double));
+    if (!rc2) r->partials = static_cast<double *>(ptr);
+    if (!rc2) rc2 = alloc_dev(r, &ptr, 4 * sizeof(double));
+    if (!rc2) r->sums4 = static_cast<double *>(ptr);
+    if (rc2) {
+      const std::string msg = g_last_error;
+      e2d_destroy(r);
+      return fail(rc2, msg);
+    }
+    // defined contents everywhere (ghosts included) before the first init
+    const size_t bytes = (size_t)r->U.plane * 4 * sizeof(double);
+    HIP_TRY(hipMemsetAsync(r->U.p, 0, bytes, r->stream));
+    HIP_TRY(hipMemsetAsync(r->U2.p, 0, bytes, r->stream));
+    HIP_TRY(hipMemsetAsync(r->ctl, 0, sizeof(e2d::StepCtl), r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    *out = r;
+    return E2D_OK;
+  });
+}
+
+int e2d_destroy(e2d_run *run) {
+  if (!run) return fail(E2D_ERR_ARG, "null e2d_run handle");
+  (void)hipSetDevice(run->device);
+  if (run->stream) (void)hipStreamSynchronize(run->stream);
+  for (void *ptr : run->allocs) (void)hipFree(ptr);
+  if (run->stream) (void)hipStreamDestroy(run->stream);
+  delete run;
+  return E2D_OK;
+}
+
+int e2d_init_problem(e2d_run *r) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  const bool sedov = r->p.problemType == 1 && r->p.blast_total_energy_inside > 0;
+  if (sedov) HIP_TRY(hipMemsetAsync(r->count, 0, sizeof(unsigned int), r->stream));
+  HIP_TRY(e2d::launch_init(r->p, r->U, sedov ? r->count : nullptr, r->stream));
+  if (sedov) {
+    // volume inside the disk, accumulated in a fixed sequential order
+    unsigned int n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, r->count, sizeof(n), hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    double volume = 0.0;
+    for (unsigned int k = 0; k < n; ++k) volume += r->p.dx * r->p.dy;
+    HIP_TRY(e2d::launch_blast_energy(r->p, r->U, r->p.blast_total_energy_inside / volume,
+                                     r->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(r->U2.p, r->U.p, (size_t)r->U.plane * 4 * sizeof(double),
+                         hipMemcpyDeviceToDevice, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return E2D_OK;
+}
+
+int e2d_compute_dt(e2d_run *r, int useU, double *dt) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (!dt) return fail(E2D_ERR_ARG, "null dt");
+  HIP_TRY(hipMemsetAsync(r->bits, 0, sizeof(unsigned long long), r->stream));
+  HIP_TRY(e2d::launch_dt_reduce(r->kp, useU == 0 ? r->U : r->U2, r->bits, r->stream));
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, r->bits, sizeof(bits), hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  double invdt;
+  std::memcpy(&invdt, &bits, sizeof(invdt));
+  *dt = r->p.cfl / invdt;  // HydroRun.h:246
+  return E2D_OK;
+}
+
+int e2d_make_boundaries(e2d_run *r, int which) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (which != E2D_U && which != E2D_U2) return fail(E2D_ERR_ARG, "which must be E2D_U or E2D_U2");
+  HIP_TRY(e2d::launch_boundaries(r->kp, state(r, which), r->stream));
+  return E2D_OK;
+}
+
+int e2d_godunov_unsplit(e2d_run *r, int nStep, double dt) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (nStep < 0) return fail(E2D_ERR_ARG, "nStep must be >= 0");
+  return do_step(r, nStep, dt, true);
+}
+
+int e2d_run_loop(e2d_run *r, int max_steps, double *t, int *nStep) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  return run_loop_impl(r, max_steps, t, nStep, nullptr, nullptr);
+}
+
+int e2d_time_loop(e2d_run *r, int nsteps, double *elapsed_ms, int *executed, double *t,
+                  int *nStep) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (nsteps <= 0) return fail(E2D_ERR_ARG, "nsteps must be > 0");
+  return run_loop_impl(r, nsteps, t, nStep, elapsed_ms, executed);
+}
+
+int e2d_copy_state(e2d_run *r, int which, double *host, int layout) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (!host || (which != E2D_U && which != E2D_U2)) return fail(E2D_ERR_ARG, "bad argument");
+  const Grid &g = state(r, which);
+  const int isz = r->p.isize, jsz = r->p.jsize, nx = r->p.nx, ny = r->p.ny;
+  return guarded([&]() -> int {
+    if (layout == E2D_LAYOUT_INTERIOR) {
+      for (int v = 0; v < 4; v++)
+        HIP_TRY(hipMemcpy2DAsync(host + (size_t)v * nx * ny, (size_t)nx * sizeof(double),
+                                 g.p + v * g.plane + 2LL * g.pitch + 2,
+                                 (size_t)g.pitch * sizeof(double), (size_t)nx * sizeof(double),
+                                 (size_t)ny, hipMemcpyDeviceToHost, r->stream));
+      HIP_TRY(hipStreamSynchronize(r->stream));
+      return E2D_OK;
+    }
+    if (layout != E2D_LAYOUT_LEFT && layout != E2D_LAYOUT_RIGHT)
+      return fail(E2D_ERR_ARG, "unknown layout");
+    std::vector<double> tmp;
+    double *dst = host;
+    if (layout == E2D_LAYOUT_RIGHT) {
+      tmp.resize((size_t)isz * jsz * 4);
+      dst = tmp.data();
+    }
+    for (int v = 0; v < 4; v++)
+      HIP_TRY(hipMemcpy2DAsync(dst + (size_t)v * isz * jsz, (size_t)isz * sizeof(double),
+                               g.p + v * g.plane, (size_t)g.pitch * sizeof(double),
+                               (size_t)isz * sizeof(double), (size_t)jsz, hipMemcpyDeviceToHost,
+                               r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    if (layout == E2D_LAYOUT_RIGHT)
+      for (int i = 0; i < isz; i++)
+        for (int j = 0; j < jsz; j++)
+          for (int v = 0; v < 4; v++)
+            host[((size_t)i * jsz + j) * 4 + v] = tmp[(size_t)v * isz * jsz + (size_t)j * isz + i];
+    return E2D_OK;
+  });
+}
+
+int e2d_upload_state(e2d_run *r, int which, const double *host, int layout) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (!host || (which != E2D_U && which != E2D_U2)) return fail(E2D_ERR_ARG, "bad argument");
+  const Grid &g = state(r, which);
+  const int isz = r->p.isize, jsz = r->p.jsize, nx = r->p.nx, ny = r->p.ny;
+  return guarded([&]() -> int {
+    if (layout == E2D_LAYOUT_INTERIOR) {
+      for (int v = 0; v < 4; v++)
+        HIP_TRY(hipMemcpy2DAsync(g.p + v * g.plane + 2LL * g.pitch + 2,
+                                 (size_t)g.pitch * sizeof(double), host + (size_t)v * nx * ny,
+                                 (size_t)nx * sizeof(double), (size_t)nx * sizeof(double),
+                                 (size_t)ny, hipMemcpyHostToDevice, r->stream));
+      HIP_TRY(hipStreamSynchronize(r->stream));
+      return E2D_OK;
+    }
+    if (layout != E2D_LAYOUT_LEFT && layout != E2D_LAYOUT_RIGHT)
+      return fail(E2D_ERR_ARG, "unknown layout");
+    std::vector<double> tmp;
+    const double *src = host;
+    if (layout == E2D_LAYOUT_RIGHT) {
+      tmp.resize((size_t)isz * jsz * 4);
+      for (int i = 0; i < isz; i++)
+        for (int j = 0; j < jsz; j++)
+          for (int v = 0; v < 4; v++)
+            tmp[(size_t)v * isz * jsz + (size_t)j * isz + i] = host[((size_t)i * jsz + j) * 4 + v];
+      src = tmp.data();
+    }
+    for (int v = 0; v < 4; v++)
+      HIP_TRY(hipMemcpy2DAsync(g.p + v * g.plane, (size_t)g.pitch * sizeof(double),
+                               src + (size_t)v * isz * jsz, (size_t)isz * sizeof(double),
+                               (size_t)isz * sizeof(double), (size_t)jsz, hipMemcpyHostToDevice,
+                               r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return E2D_OK;
+  });
+}
+
+int e2d_conserved_sums(e2d_run *r, int which, double sums[4]) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (!sums || (which != E2D_U && which != E2D_U2)) return fail(E2D_ERR_ARG, "bad argument");
+  HIP_TRY(e2d::launch_sums(r->kp, state(r, which), r->partials, r->sums4, r->stream));
+  HIP_TRY(hipMemcpyAsync(sums, r->sums4, 4 * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return E2D_OK;
+}
+
+int e2d_timers(e2d_run *r, double out[5]) {
+  if (!r || !out) return fail(E2D_ERR_ARG, "null argument");
+  for (int k = 0; k < 5; k++) out[k] = r->timers[k];
+  return E2D_OK;
+}
+
+int e2d_set_schedule(e2d_run *r, int schedule) {
+  if (!r) return fail(E2D_ERR_ARG, "null e2d_run handle");
+  if (schedule != E2D_SCHEDULE_FUSED && schedule != E2D_SCHEDULE_PASSES)
+    return fail(E2D_ERR_ARG, "unknown schedule");
+  r->schedule = schedule;
+  return E2D_OK;
+}
+
+int e2d_synchronize(e2d_run *r) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return E2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,617 @@
+// kernels.hip -- FP64 HIP kernels for gfx950 (MI355X).
+//
+// Reference passes reproduced (pkestene/euler2d_kokkos src/HydroRunFunctors.h):
+//   Init*Functor :1347-1827, MakeBoundariesFunctor :1832-2030, ComputeDtFunctor
+//   :17-79, ConvertToPrimitivesFunctor :84-143, ComputeAndStoreFluxesFunctor
+//   :412-651, UpdateFunctor :656-723.
+// step_fused_kernel performs BC -> prim -> slopes -> trace -> Riemann ->
+// update -> next-step dt in one pass over HBM, with bit-identical results.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace e2d {
+
+namespace {
+
+__device__ __forceinline__ double &at(const Grid &g, int v, int i, int j) {
+  return g.p[v * g.plane + (long long)j * g.pitch + i];
+}
+
+// Ghost-index map of MakeBoundariesFunctor (HydroRunFunctors.h:1876-2024) in
+// closed form: returns the interior index a ghost index copies from, and
+// whether the normal velocity flips sign (BC 1, reflecting).  BC 2 copies the
+// first/last interior cell, every other value is periodic.  The reference
+// fills x faces then y faces, so a corner ghost is the y rule applied to the x
+// rule: both maps compose, and the sign flips (exact) commute.
+__device__ __forceinline__ int ghost_map(int i, int n, int bclo, int bchi, bool &flip) {
+  flip = false;
+  if (i < 2) {
+    if (bclo == 1) {
+      flip = true;
+      return 3 - i;
+    }
+    if (bclo == 2) return 2;
+    return n + i;
+  }
+  if (i > n + 1) {
+    if (bchi == 1) {
+      flip = true;
+      return 2 * n + 3 - i;
+    }
+    if (bchi == 2) return n + 1;
+    return i - n;
+  }
+  return i;
+}
+
+__device__ __forceinline__ unsigned long long dbits(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off));
+  return x;
+}
+
+// ------------------------------------------------------------------ init
+__global__ void init_kernel(e2d_params p, Grid U, unsigned int *count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= p.isize || j >= p.jsize) return;
+  const int gw = p.ghostWidth;
+  const double gamma0 = p.gamma0;
+  const double x = p.xmin + p.dx / 2 + (i - gw) * p.dx;
+  const double y = p.ymin + p.dy / 2 + (j - gw) * p.dy;
+  double d, e, mx = 0.0, my = 0.0;
+  if (p.problemType == 1) {
+    // InitBlastFunctor, TagRegularInit (:1469-1505)
+    const double radius2 = p.blast_radius * p.blast_radius;
+    const double d2 = (x - p.blast_center_x) * (x - p.blast_center_x) +
+                      (y - p.blast_center_y) * (y - p.blast_center_y);
+    if (d2 < radius2) {
+      d = p.blast_density_in;
+      e = p.blast_pressure_in / (gamma0 - 1.0);
+      if (count) atomicAdd(count, 1u);
+    } else {
+      d = p.blast_density_out;
+      e = p.blast_pressure_out / (gamma0 - 1.0);
+    }
+  } else if (p.problemType == 2) {
+    // InitFourQuadrantFunctor (:1549-1641), primToCons :1578-1591
+    const double W[4][4] = {{1.5, 1.5, 0.0, 0.0},
+                            {0.5323, 0.3, 1.206, 0.0},
+                            {0.138, 0.029, 1.206, 1.206},
+                            {0.5323, 0.3, 0.0, 1.206}};
+    const int k = (x < 0.8) ? ((y < 0.8) ? 2 : 1) : ((y < 0.8) ? 3 : 0);
+    const double rho = W[k][0], pr = W[k][1], u = W[k][2], v = W[k][3];
+    d = rho;
+    mx = u * rho;
+    my = v * rho;
+    e = pr / (gamma0 - 1.0) + rho * (u * u + v * v) * 0.5;
+  } else if (p.problemType == 3) {
+    // InitDiscontinuityFunctor (:1673-1707)
+    if (x + y < 1) {
+      d = 1.0 + x * x;
+      e = 1.0 / (gamma0 - 1.0);
+    } else {
+      d = 0.25;
+      e = 1.0 / (gamma0 - 1.0);
+    }
+  } else if (p.problemType == 4) {
+    // InitShockedBubbleFunctor (:1741-1820)
+    if (x < p.sb_shock_loc) {
+      d = p.sb_postshock_density;
+      mx = p.sb_postshock_density * p.sb_postshock_velocity;
+      const double rho_eint = p.sb_postshock_pressure / (gamma0 - 1);
+      e = rho_eint + 0.5 * (mx * mx + my * my) / d;
+    } else {
+      const double radius = sqrt((x - p.sb_bubble_center_x) * (x - p.sb_bubble_center_x) +
+                                 (y - p.sb_bubble_center_y) * (y - p.sb_bubble_center_y));
+      if (radius < p.sb_bubble_radius) {
+        d = p.sb_bubble_density;
+        const double rho_eint = p.sb_bubble_pressure / (gamma0 - 1);
+        e = rho_eint + 0.5 * (mx * mx + my * my) / d;
+      } else {
+        d = p.sb_preshock_density;
+        const double rho_eint = p.sb_preshock_pressure / (gamma0 - 1);
+        e = rho_eint + 0.5 * (mx * mx + my * my) / d;
+      }
+    }
+  } else {
+    // InitImplodeFunctor (:1371-1397); high-state pressure 1 unless overridden
+    const double tmp = x + y * y;
+    if (tmp > 0.5 && tmp < 1.5) {
+      d = 1.0;
+      e = p.implode_pressure_in / (gamma0 - 1.0);
+    } else {
+      d = 0.125;
+      e = 0.14 / (gamma0 - 1.0);
+    }
+  }
+  at(U, ID, i, j) = d;
+  at(U, IP, i, j) = e;
+  at(U, IU, i, j) = mx;
+  at(U, IV, i, j) = my;
+}
+
+// InitBlastFunctor, TagCorrectTotalEnergy (:1507-1523)
+__global__ void blast_energy_kernel(e2d_params p, Grid U, double e_inside) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= p.isize || j >= p.jsize) return;
+  const double x = p.xmin + p.dx / 2 + (i - p.ghostWidth) * p.dx;
+  const double y = p.ymin + p.dy / 2 + (j - p.ghostWidth) * p.dy;
+  const double radius2 = p.blast_radius * p.blast_radius;
+  const double d2 = (x - p.blast_center_x) * (x - p.blast_center_x) +
+                    (y - p.blast_center_y) * (y - p.blast_center_y);
+  if (d2 < radius2) at(U, IP, i, j) = e_inside;
+}
+
+// --------------------------------------------------------------- ghosts
+// One thread per ghost cell: 2 rows at the bottom and top (all columns), then
+// 2 columns on each side of every interior row.
+__global__ void boundaries_kernel(KParams kp, Grid U) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rows = 4 * kp.isize;
+  int i, j;
+  if (t < rows) {
+    const int r = t / kp.isize;
+    i = t - r * kp.isize;
+    j = r < 2 ? r : kp.ny + r;  // rows 0,1,ny+2,ny+3
+  } else {
+    const int t2 = t - rows;
+    if (t2 >= 4 * kp.ny) return;
+    j = 2 + (t2 >> 2);
+    const int c = t2 & 3;
+    i = c < 2 ? c : kp.nx + c;  // columns 0,1,nx+2,nx+3
+  }
+  bool fx, fy;
+  const int i0 = ghost_map(i, kp.nx, kp.bcxmin, kp.bcxmax, fx);
+  const int j0 = ghost_map(j, kp.ny, kp.bcymin, kp.bcymax, fy);
+  const double d = at(U, ID, i0, j0);
+  const double e = at(U, IP, i0, j0);
+  double mx = at(U, IU, i0, j0);
+  double my = at(U, IV, i0, j0);
+  if (fx) mx = -mx;
+  if (fy) my = -my;
+  at(U, ID, i, j) = d;
+  at(U, IP, i, j) = e;
+  at(U, IU, i, j) = mx;
+  at(U, IV, i, j) = my;
+}
+
+// ------------------------------------------------------------------- dt
+__global__ void dt_reduce_kernel(KParams kp, Grid U, unsigned long long *invdt_bits) {
+  const long long n = (long long)kp.nx * kp.ny;
+  double m = 0.0;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int j = 2 + (int)(idx / kp.nx);
+    const int i = 2 + (int)(idx - (long long)(j - 2) * kp.nx);
+    m = fmax(m, cfl_rate(kp.s, at(U, ID, i, j), at(U, IP, i, j), at(U, IU, i, j),
+                         at(U, IV, i, j), kp.dx, kp.dy));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0 && m == m) atomicMax(invdt_bits, dbits(m));
+}
+
+// Starts a run loop: dt of the pending step from the reduced invDt, with the
+// tEnd clamp and the while condition of main.cpp:100,128-134.
+__global__ void dt_finalize_kernel(KParams kp, StepCtl *ctl) {
+  const double invdt = __longlong_as_double((long long)ctl->invdt);
+  const double t = ctl->t;
+  double dt = kp.cfl / invdt;
+  if (t + dt > kp.tEnd) dt = kp.tEnd - t;
+  ctl->dt = dt;
+  ctl->active = (t < kp.tEnd && ctl->nstep < kp.nStepmax) ? 1 : 0;
+  ctl->invdt = 0ull;
+  ctl->blocks_done = 0u;
+}
+
+// ------------------------------------------------ reference pass schedule
+__global__ void prim_kernel(KParams kp, Grid U, Grid Q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= kp.isize || j >= kp.jsize) return;
+  Q4 q;
+  prim_no_c(kp.s, at(U, ID, i, j), at(U, IP, i, j), at(U, IU, i, j), at(U, IV, i, j), q);
+  for (int v = 0; v < 4; v++) at(Q, v, i, j) = q.v[v];
+}
+
+__device__ __forceinline__ Q4 ld4(const Grid &g, int i, int j) {
+  Q4 q;
+  for (int v = 0; v < 4; v++) q.v[v] = at(g, v, i, j);
+  return q;
+}
+
+__device__ __forceinline__ void slopes(const Scheme &s, const Q4 &q, const Q4 &px, const Q4 &mx,
+                                       const Q4 &py, const Q4 &my, Q4 &dqx, Q4 &dqy) {
+  for (int v = 0; v < 4; v++) {
+    dqx.v[v] = slope1(s, q.v[v], px.v[v], mx.v[v]);
+    dqy.v[v] = slope1(s, q.v[v], py.v[v], my.v[v]);
+  }
+}
+
+// ComputeAndStoreFluxesFunctor (:449-644), i,j in [2, size-2]
+__global__ void fluxes_kernel(KParams kp, Grid Q, Grid Fx, Grid Fy, double dtdx, double dtdy) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > kp.isize - 2 || j > kp.jsize - 2) return;
+  const Scheme &s = kp.s;
+  const Q4 qc = ld4(Q, i, j);
+  Q4 dqX, dqY, dqXn, dqYn, fr, fl;
+  slopes(s, qc, ld4(Q, i + 1, j), ld4(Q, i - 1, j), ld4(Q, i, j + 1), ld4(Q, i, j - 1), dqX, dqY);
+  const Sources sc = trace_sources(s, qc, dqX, dqY);
+  double flux[4];
+  {
+    const Q4 qn = ld4(Q, i - 1, j);
+    slopes(s, qn, qc, ld4(Q, i - 2, j), ld4(Q, i - 1, j + 1), ld4(Q, i - 1, j - 1), dqXn, dqYn);
+    const Sources sn = trace_sources(s, qn, dqXn, dqYn);
+    trace_face(s, qc, dqX, sc, dtdx, false, fr);
+    trace_face(s, qn, dqXn, sn, dtdx, true, fl);
+    riemann(s, fl, fr, IU, IV, flux);
+    for (int v = 0; v < 4; v++) at(Fx, v, i, j) = flux[v] * dtdx;
+  }
+  {
+    const Q4 qn = ld4(Q, i, j - 1);
+    slopes(s, qn, ld4(Q, i + 1, j - 1), ld4(Q, i - 1, j - 1), qc, ld4(Q, i, j - 2), dqXn, dqYn);
+    const Sources sn = trace_sources(s, qn, dqXn, dqYn);
+    trace_face(s, qc, dqY, sc, dtdy, false, fr);
+    trace_face(s, qn, dqYn, sn, dtdy, true, fl);
+    riemann(s, fl, fr, IV, IU, flux);
+    for (int v = 0; v < 4; v++) at(Fy, v, i, j) = flux[v] * dtdy;
+  }
+}
+
+// UpdateFunctor (:683-719)
+__global__ void update_kernel(KParams kp, Grid U, Grid Fx, Grid Fy) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > kp.nx + 1 || j > kp.ny + 1) return;
+  for (int v = 0; v < 4; v++) {
+    double u = at(U, v, i, j);
+    u += at(Fx, v, i, j);
+    u -= at(Fx, v, i + 1, j);
+    u += at(Fy, v, i, j);
+    u -= at(Fy, v, i, j + 1);
+    at(U, v, i, j) = u;
+  }
+}
+
+// ------------------------------------------------------ fused step kernel
+//
+// One wavefront owns a strip of 64 columns (60 output columns + 2 halo columns
+// per side) and marches up `seg_rows` output rows.  Per loaded row r it
+// converts U to primitives once, then for row a = r-1 computes the slopes
+// once, the Hancock source terms once and the face states it needs, the x-face
+// fluxes of row a and the y-face flux between rows a-1 and a, and finally
+// updates row b = r-2 with the reference's operation order
+// (((U + Fx_i) - Fx_{i+1}) + Fy_j) - Fy_{j+1}.  Horizontal neighbours come from
+// lane shuffles, vertical ones from registers carried between iterations, so
+// nothing is staged through LDS and every cell is read from HBM once (plus
+// the 4-row / 4-column halo overlap between neighbouring waves).
+// Ghost cells are produced on the fly from the closed-form BC map; the wave
+// owning a ghost cell also writes it to `in` and `out`, as make_boundaries(in)
+// followed by deep_copy(out, in) leaves them in the reference.
+// In run-loop mode (ctl != nullptr) the kernel reads dt from the control
+// block, folds the next step's CFL rate of every updated cell into a wave max,
+// and the last block to finish advances t, nStep and the next dt.
+
+__device__ __forceinline__ Q4 shfl_up1(const Q4 &a) {
+  Q4 r;
+  for (int v = 0; v < 4; v++) r.v[v] = __shfl_up(a.v[v], 1);
+  return r;
+}
+
+__device__ __forceinline__ Q4 shfl_dn1(const Q4 &a) {
+  Q4 r;
+  for (int v = 0; v < 4; v++) r.v[v] = __shfl_down(a.v[v], 1);
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in, Grid out,
+                                                            double dt_arg, StepCtl *ctl,
+                                                            int seg_rows, int nstrips, int nwaves) {
+  if (ctl && !ctl->active) return;  // loop finished: nothing to do (uniform)
+  const Scheme &s = kp.s;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6);
+  double wmax = 0.0;
+
+  if (wave < nwaves) {
+    const double dt = ctl ? ctl->dt : dt_arg;
+    const double dtdx = dt / kp.dx;
+    const double dtdy = dt / kp.dy;
+
+    const int strip = wave % nstrips;
+    const int seg = wave / nstrips;
+    const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
+    const int i_base = 2 + kOutCols * strip;
+    const int col = i_base - 2 + lane;
+    const int j_base = 2 + seg_rows * seg;
+
+    bool flipx;
+    const int colc = min(col, kp.nx + 3);
+    const int c0 = ghost_map(colc, kp.nx, kp.bcxmin, kp.bcxmax, flipx);
+    const bool col_ghost = col < 2 || (col > kp.nx + 1 && col <= kp.nx + 3);
+    const bool col_owned = (lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1) ||
+                           (strip == 0 && lane < 2) ||
+                           (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
+    const bool out_col = lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1;
+
+    auto load_row = [&](int r, double u[4]) {
+      bool flipy;
+      const int rc = min(r, kp.ny + 3);
+      const int r0 = ghost_map(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
+      const long long off = (long long)r0 * in.pitch + c0;
+      u[0] = in.p[off];
+      u[1] = in.p[in.plane + off];
+      u[2] = in.p[2 * in.plane + off];
+      u[3] = in.p[3 * in.plane + off];
+      if (flipx) u[2] = -u[2];
+      if (flipy) u[3] = -u[3];
+    };
+
+    const int nrows = seg_rows + 4;
+    double un[4], ua[4] = {0, 0, 0, 0}, ub[4] = {0, 0, 0, 0};
+    Q4 qm2 = {}, qm1 = {}, fymax_c = {}, fx_c = {}, fy_c = {};
+    load_row(j_base - 2, un);
+
+    for (int k = 0; k < nrows; ++k) {
+      const int r = j_base - 2 + k;
+      double uc[4] = {un[0], un[1], un[2], un[3]};
+      if (k + 1 < nrows) load_row(r + 1, un);
+
+      // ghost cells of this row owned by this wave: materialise them in both arrays
+      const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
+      const bool row_owned = (r >= j_base && r < j_base + seg_rows && r <= kp.ny + 1) ||
+                             (seg == 0 && r < 2) ||
+                             (seg == nsegs - 1 && r > kp.ny + 1 && r <= kp.ny + 3);
+      if ((col_ghost || row_ghost) && col_owned && row_owned) {
+        const long long off = (long long)r * in.pitch + col;
+        for (int v = 0; v < 4; v++) {
+          in.p[v * in.plane + off] = uc[v];
+          out.p[v * out.plane + off] = uc[v];
+        }
+      }
+
+      Q4 q0;
+      prim_no_c(s, uc[0], uc[1], uc[2], uc[3], q0);
+
+      if (k >= 2) {
+        // row a = r - 1: slopes, source terms, y-face states
+        const Q4 qpx = shfl_dn1(qm1);
+        const Q4 qmx = shfl_up1(qm1);
+        Q4 dqX, dqY;
+        for (int v = 0; v < 4; v++) {
+          dqX.v[v] = slope1(s, qm1.v[v], qpx.v[v], qmx.v[v]);
+          dqY.v[v] = slope1(s, qm1.v[v], q0.v[v], qm2.v[v]);
+        }
+        const Sources src = trace_sources(s, qm1, dqX, dqY);
+        Q4 fymin, fymax;
+        trace_face(s, qm1, dqY, src, dtdy, false, fymin);
+        trace_face(s, qm1, dqY, src, dtdy, true, fymax);
+        if (k >= 3) {
+          // y face between rows a-1 and a
+          double f[4];
+          riemann(s, fymax_c, fymin, IV, IU, f);
+          Q4 fya;
+          for (int v = 0; v < 4; v++) fya.v[v] = f[v] * dtdy;
+          Q4 fxa = {};
+          if (k <= seg_rows + 2) {
+            // x faces of row a: left state from lane-1's XMAX face
+            Q4 fxmin, fxmax;
+            trace_face(s, qm1, dqX, src, dtdx, false, fxmin);
+            trace_face(s, qm1, dqX, src, dtdx, true, fxmax);
+            const Q4 ql = shfl_up1(fxmax);
+            riemann(s, ql, fxmin, IU, IV, f);
+            for (int v = 0; v < 4; v++) fxa.v[v] = f[v] * dtdx;
+          }
+          if (k >= 4) {
+            // update row b = a - 1
+            const int b = r - 2;
+            const Q4 fxr = shfl_dn1(fx_c);
+            double un4[4];
+            for (int v = 0; v < 4; v++) {
+              double u = ub[v];
+              u += fx_c.v[v];
+              u -= fxr.v[v];
+              u += fy_c.v[v];
+              u -= fya.v[v];
+              un4[v] = u;
+            }
+            if (out_col && b <= kp.ny + 1) {
+              const long long off = (long long)b * out.pitch + col;
+              for (int v = 0; v < 4; v++) out.p[v * out.plane + off] = un4[v];
+              if (ctl) wmax = fmax(wmax, cfl_rate(s, un4[0], un4[1], un4[2], un4[3], kp.dx, kp.dy));
+            }
+          }
+          fx_c = fxa;
+          fy_c = fya;
+        }
+        fymax_c = fymax;
+      }
+      qm2 = qm1;
+      qm1 = q0;
+      for (int v = 0; v < 4; v++) {
+        ub[v] = ua[v];
+        ua[v] = uc[v];
+      }
+    }
+  }
+
+  if (!ctl) return;
+  // run-loop epilogue: next-step invDt (exact max, order independent) and the
+  // last block advances the loop state
+  wmax = wave_max(wmax);
+  if (lane == 0 && wmax == wmax && wmax > 0.0) atomicMax(&ctl->invdt, dbits(wmax));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned int prev = atomicAdd(&ctl->blocks_done, 1u);
+    if (prev == gridDim.x - 1) {
+      __threadfence();
+      const unsigned long long bits = atomicMax(&ctl->invdt, 0ull);
+      const double invdt = __longlong_as_double((long long)bits);
+      const double t = ctl->t + ctl->dt;  // t += dt (main.cpp:143)
+      const int n = ctl->nstep + 1;
+      double dtn = kp.cfl / invdt;       // HydroRun.h:246
+      if (t + dtn > kp.tEnd) dtn = kp.tEnd - t;  // main.cpp:131-134
+      ctl->t = t;
+      ctl->nstep = n;
+      ctl->dt = dtn;
+      ctl->active = (t < kp.tEnd && n < kp.nStepmax) ? 1 : 0;
+      atomicExch(&ctl->invdt, 0ull);
+      atomicExch(&ctl->blocks_done, 0u);
+    }
+  }
+}
+
+// -------------------------------------------------- conserved-variable sums
+// Deterministic two-level sum: block b sums rows b, b+G, ... in a fixed tree.
+constexpr int kSumBlocks = 512;
+
+__global__ void sums_partial_kernel(KParams kp, Grid U, double *partials) {
+  __shared__ double red[4][256];
+  double acc[4] = {0, 0, 0, 0};
+  for (int j = 2 + blockIdx.x; j <= kp.ny + 1; j += gridDim.x)
+    for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x)
+      for (int v = 0; v < 4; v++) acc[v] += at(U, v, i, j);
+  for (int v = 0; v < 4; v++) red[v][threadIdx.x] = acc[v];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int v = 0; v < 4; v++) red[v][threadIdx.x] += red[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) partials[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void sums_final_kernel(const double *partials, int nblocks, double *out4) {
+  __shared__ double red[4][256];
+  double acc[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int v = 0; v < 4; v++) acc[v] += partials[b * 4 + v];
+  for (int v = 0; v < 4; v++) red[v][threadIdx.x] = acc[v];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int v = 0; v < 4; v++) red[v][threadIdx.x] += red[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out4[threadIdx.x] = red[threadIdx.x][0];
+}
+
+dim3 grid2d(int nx, int ny, dim3 blk) {
+  return dim3((unsigned)((nx + blk.x - 1) / blk.x), (unsigned)((ny + blk.y - 1) / blk.y));
+}
+
+}  // namespace
+
+KParams make_kparams(const e2d_params &p) {
+  KParams k{};
+  k.s.gamma0 = p.gamma0;
+  k.s.gamma6 = p.gamma6;
+  k.s.smallr = p.smallr;
+  k.s.smallc = p.smallc;
+  k.s.smallp = p.smallp;
+  k.s.smallpp = p.smallpp;
+  k.s.slope_type = p.slope_type;
+  k.s.entho = 1.0 / (p.gamma0 - 1.0);
+  k.s.smallc2 = p.smallc * p.smallc;
+  k.s.riemann = p.honor_riemann ? p.riemannSolverType : 2;
+  k.nx = p.nx;
+  k.ny = p.ny;
+  k.isize = p.isize;
+  k.jsize = p.jsize;
+  k.bcxmin = p.boundary_type_xmin;
+  k.bcxmax = p.boundary_type_xmax;
+  k.bcymin = p.boundary_type_ymin;
+  k.bcymax = p.boundary_type_ymax;
+  k.nStepmax = p.nStepmax;
+  k.dx = p.dx;
+  k.dy = p.dy;
+  k.cfl = p.cfl;
+  k.tEnd = p.tEnd;
+  return k;
+}
+
+hipError_t launch_init(const e2d_params &p, Grid U, unsigned int *count_dev, hipStream_t st) {
+  const dim3 blk(64, 4);
+  hipLaunchKernelGGL(init_kernel, grid2d(p.isize, p.jsize, blk), blk, 0, st, p, U, count_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_blast_energy(const e2d_params &p, Grid U, double e_inside, hipStream_t st) {
+  const dim3 blk(64, 4);
+  hipLaunchKernelGGL(blast_energy_kernel, grid2d(p.isize, p.jsize, blk), blk, 0, st, p, U,
+                     e_inside);
+  return hipGetLastError();
+}
+
+hipError_t launch_boundaries(const KParams &kp, Grid U, hipStream_t st) {
+  const int n = 4 * kp.isize + 4 * kp.ny;
+  hipLaunchKernelGGL(boundaries_kernel, dim3((n + 255) / 256), dim3(256), 0, st, kp, U);
+  return hipGetLastError();
+}
+
+hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt_bits,
+                            hipStream_t st) {
+  const long long n = (long long)kp.nx * kp.ny;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(dt_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U,
+                     invdt_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_dt_finalize(const KParams &kp, StepCtl *ctl, hipStream_t st) {
+  hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, st, kp, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_prim(const KParams &kp, Grid U, Grid Q, hipStream_t st) {
+  const dim3 blk(64, 4);
+  hipLaunchKernelGGL(prim_kernel, grid2d(kp.isize, kp.jsize, blk), blk, 0, st, kp, U, Q);
+  return hipGetLastError();
+}
+
+hipError_t launch_fluxes(const KParams &kp, Grid Q, Grid Fx, Grid Fy, double dtdx, double dtdy,
+                         hipStream_t st) {
+  const dim3 blk(64, 4);
+  hipLaunchKernelGGL(fluxes_kernel, grid2d(kp.nx + 1, kp.ny + 1, blk), blk, 0, st, kp, Q, Fx, Fy,
+                     dtdx, dtdy);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const KParams &kp, Grid U, Grid Fx, Grid Fy, hipStream_t st) {
+  const dim3 blk(64, 4);
+  hipLaunchKernelGGL(update_kernel, grid2d(kp.nx, kp.ny, blk), blk, 0, st, kp, U, Fx, Fy);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl,
+                             int seg_rows, hipStream_t st) {
+  if (seg_rows < 1) seg_rows = 1;
+  const int nstrips = (kp.nx + kOutCols - 1) / kOutCols;
+  const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
+  const int nwaves = nstrips * nsegs;
+  const int waves_per_block = kBlock / kLanes;
+  const int blocks = (nwaves + waves_per_block - 1) / waves_per_block;
+  hipLaunchKernelGGL(step_fused_kernel, dim3(blocks), dim3(kBlock), 0, st, kp, in, out, dt, ctl,
+                     seg_rows, nstrips, nwaves);
+  return hipGetLastError();
+}
+
+int sums_partials_needed(const KParams &) { return kSumBlocks * 4; }
+
+hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st) {
+  hipLaunchKernelGGL(sums_partial_kernel, dim3(kSumBlocks), dim3(256), 0, st, kp, U, partials);
+  hipLaunchKernelGGL(sums_final_kernel, dim3(1), dim3(256), 0, st, partials, kSumBlocks, out4);
+  return hipGetLastError();
+}
+
+}  // namespace e2d

@@ -1,0 +1,195 @@
+/*
+ * euler2d_hip.h -- C ABI of the MI355X-native unsplit MUSCL-Hancock Godunov
+ * solver (libeuler2d_hip.so).
+ *
+ * This is the drop-in boundary for the reference's solver API,
+ * pkestene/euler2d_kokkos:
+ *   - HydroParams / HydroSettings + ConfigMap .ini parsing
+ *       (src/HydroParams.h:107-265, src/HydroParams.cpp:19-224,
+ *        config/ConfigMap.h:25-47, config/inih/INIReader.h:47-67)
+ *   - class HydroRun<device_t> (src/HydroRun.h:44-134): constructor and
+ *     problem initialisation (:142-216), compute_dt (:227-251),
+ *     godunov_unsplit (:258-364), make_boundaries (:388-399)
+ *   - the time loop of main() (src/main.cpp:87-145)
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative E2D_ERR_* code on
+ *     failure; nothing calls exit() or throws across the ABI.  The message of
+ *     the last failure on the calling thread is returned by e2d_last_error().
+ *   - Reals are double; every real-valued .ini parameter is parsed as float
+ *     (strtof) and then widened, as the reference does.
+ *   - A handle owns its device buffers and one HIP stream.  Calls on one handle
+ *     must be serialised by the caller; different handles may be used from
+ *     different threads.
+ *   - Variable order in every state array: 0 rho, 1 E, 2 rho*u, 3 rho*v
+ *     (src/HydroParams.h:27-34, ID=0, IP=IE=1, IU=2, IV=3).
+ */
+#ifndef EULER2D_HIP_H_
+#define EULER2D_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define E2D_OK 0
+#define E2D_ERR_ARG (-1)         /* invalid argument or handle */
+#define E2D_ERR_IO (-2)          /* file could not be opened */
+#define E2D_ERR_HIP (-3)         /* HIP runtime failure / no device */
+#define E2D_ERR_ALLOC (-4)       /* device or host allocation failed */
+#define E2D_ERR_UNSUPPORTED (-5) /* parameter combination not supported */
+
+/* state selectors (HydroRun members U, U2) */
+#define E2D_U 0
+#define E2D_U2 1
+
+/* host array layouts for e2d_copy_state / e2d_upload_state */
+#define E2D_LAYOUT_RIGHT 0     /* [i][j][var], ghosts included (reference OpenMP View layout) */
+#define E2D_LAYOUT_LEFT 1      /* [var][j][i], ghosts included (reference CUDA/HIP View layout) */
+#define E2D_LAYOUT_INTERIOR 2  /* [var][j][i], interior nx*ny only */
+
+/* numeric precision of a run */
+#define E2D_PRECISION_FP64 0
+#define E2D_PRECISION_FP32_SPLIT 1
+
+/* step schedules of the FP64 path */
+#define E2D_SCHEDULE_FUSED 0    /* one fused kernel per step (default) */
+#define E2D_SCHEDULE_PASSES 1   /* reference pass structure: BC, copy, prim, fluxes, update */
+
+/*
+ * POD mirror of HydroParams + HydroSettings + blast + shocked-bubble fields
+ * (src/HydroParams.h:107-265).  Derived fields (imax, jmax, isize, jsize, dx,
+ * dy, gamma6, smallp, smallpp) are filled by e2d_params_init().
+ * Two extension fields, whose defaults keep the reference's behaviour:
+ *   honor_riemann       0: HLLC always (the reference ignores `riemann`,
+ *                       src/HydroRunFunctors.h:567,631); 1: use riemannSolverType
+ *                       ([hydro] honor_riemann)
+ *   implode_pressure_in high-state pressure of the implode problem
+ *                       ([implode] pressure_in, default 1; the reference
+ *                       hard-codes 1, src/HydroRunFunctors.h:1390)
+ */
+typedef struct e2d_params {
+  /* [run] */
+  int32_t nStepmax;
+  int32_t nOutput;
+  int32_t enableOutput;
+  int32_t ioVTK;
+  int32_t ioHDF5;
+  int32_t ghostWidth; /* always 2 */
+  double tEnd;
+  /* [mesh] */
+  int32_t nx, ny;
+  int32_t imin, imax, jmin, jmax;
+  int32_t isize, jsize;
+  int32_t boundary_type_xmin, boundary_type_xmax; /* 1 reflect, 2 zero-gradient, other periodic */
+  int32_t boundary_type_ymin, boundary_type_ymax;
+  double xmin, xmax, ymin, ymax, dx, dy;
+  /* [hydro] */
+  double gamma0, gamma6, cfl, slope_type, smallr, smallc, smallp, smallpp;
+  int32_t niter_riemann;
+  int32_t riemannSolverType; /* 0 approx, 1 hll, 2 hllc */
+  int32_t problemType;       /* 0 implode, 1 blast, 2 four_quadrant, 3 discontinuity, 4 shocked_bubble */
+  int32_t implementationVersion;
+  int32_t honor_riemann;
+  int32_t blast_nbins;
+  /* [blast] */
+  double blast_radius, blast_center_x, blast_center_y;
+  double blast_density_in, blast_density_out;
+  double blast_pressure_in, blast_pressure_out;
+  double blast_total_energy_inside;
+  /* [shocked_bubble] (ShockedBubbleParams, src/HydroParams.h:134-147) */
+  double sb_bubble_radius, sb_bubble_center_x, sb_bubble_center_y;
+  double sb_bubble_density, sb_bubble_pressure;
+  double sb_preshock_density, sb_preshock_pressure;
+  double sb_postshock_density, sb_postshock_pressure, sb_postshock_velocity;
+  double sb_shock_loc;
+  /* [implode] */
+  double implode_pressure_in;
+} e2d_params;
+
+typedef struct e2d_run e2d_run;       /* opaque solver handle (HydroRun) */
+typedef struct e2d_config e2d_config; /* opaque parsed .ini (ConfigMap) */
+
+/* ---------------------------------------------------------------- library */
+const char *e2d_last_error(void);
+int e2d_version(void);
+int e2d_params_size(void); /* sizeof(e2d_params), for FFI layout checks */
+/* name of device 0 and its CU count; E2D_ERR_HIP when no GPU is visible */
+int e2d_device_info(char *name, int name_len, int *num_cu);
+
+/* ------------------------------------------------------- ConfigMap (.ini) */
+/* ConfigMap(std::string) -- config/ConfigMap.cpp:20-23, INIReader.cpp:17-20.
+ * parse_error receives ini_parse's result: 0, the first bad line, or -1 when
+ * the file cannot be opened (the reference then silently uses defaults). */
+int e2d_config_open(const char *path, e2d_config **out, int *parse_error);
+int e2d_config_close(e2d_config *cfg);
+/* INIReader::getString -- INIReader.cpp:36-42; returns the full length */
+int e2d_config_get_string(const e2d_config *cfg, const char *section, const char *name,
+                          const char *default_value, char *out, int out_len);
+/* INIReader::getInteger -- INIReader.cpp:54-64 (strtol, base 0) */
+int e2d_config_get_integer(const e2d_config *cfg, const char *section, const char *name,
+                           long default_value, long *out);
+/* ConfigMap::getFloat -- config/ConfigMap.cpp:31-40 (strtof) */
+int e2d_config_get_float(const e2d_config *cfg, const char *section, const char *name,
+                         float default_value, float *out);
+/* ConfigMap::getBool -- config/ConfigMap.cpp:57-76 */
+int e2d_config_get_bool(const e2d_config *cfg, const char *section, const char *name,
+                        int default_value, int *out);
+
+/* ------------------------------------------------------------ HydroParams */
+/* HydroParams::setup(ConfigMap&) -- src/HydroParams.cpp:42-156 */
+int e2d_params_setup(const e2d_config *cfg, e2d_params *out);
+/* ConfigMap(path) + HydroParams::setup in one call (parse_error may be NULL) */
+int e2d_params_from_ini(const char *path, e2d_params *out, int *parse_error);
+/* HydroParams::init -- src/HydroParams.cpp:160-190 (derived fields) */
+int e2d_params_init(e2d_params *p);
+/* HydroParams::print -- src/HydroParams.cpp:195-224 */
+int e2d_params_print(const e2d_params *p);
+/* checks what the kernels assume: nx,ny >= 2, ghostWidth 2, dx,dy > 0,
+ * gamma0 > 1, slope_type in {0,1,2}, riemannSolverType in {0,1,2} */
+int e2d_params_validate(const e2d_params *p);
+
+/* --------------------------------------------------------------- HydroRun */
+/* HydroRun(HydroParams&, ConfigMap&) -- src/HydroRun.h:142-216, minus the
+ * problem initialisation (see e2d_init_problem).  Copies the params. */
+int e2d_create(const e2d_params *p, int precision, e2d_run **out);
+int e2d_destroy(e2d_run *run);
+/* Init<Problem>Functor::apply(params, U); deep_copy(U2, U) -- HydroRun.h:185-214 */
+int e2d_init_problem(e2d_run *run);
+/* HydroRun::compute_dt(useU) -- HydroRun.h:227-251 (synchronous) */
+int e2d_compute_dt(e2d_run *run, int useU, double *dt);
+/* HydroRun::make_boundaries(U or U2) -- HydroRun.h:388-399 */
+int e2d_make_boundaries(e2d_run *run, int which);
+/* HydroRun::godunov_unsplit(nStep, dt) -- HydroRun.h:258-364: in = U for even
+ * nStep else U2, out = the other; asynchronous on the handle's stream */
+int e2d_godunov_unsplit(e2d_run *run, int nStep, double dt);
+/* The main() time loop (src/main.cpp:87-145) without output: on entry *nStep
+ * and *t hold the loop state (0, 0 for a fresh run; the startup dt and
+ * boundary fills of main.cpp:87-91 run when *nStep == 0).  Runs until
+ * t >= tEnd, nStep >= nStepmax, or max_steps iterations (max_steps <= 0: no
+ * cap).  dt, the tEnd clamp and t += dt stay on the device (no host round
+ * trip per step); returns the new t and nStep. */
+int e2d_run_loop(e2d_run *run, int max_steps, double *t, int *nStep);
+/* device <-> host copies of U / U2 in one of the E2D_LAYOUT_* layouts */
+int e2d_copy_state(e2d_run *run, int which, double *host, int layout);
+int e2d_upload_state(e2d_run *run, int which, const double *host, int layout);
+/* FP64 interior sums of the four conserved variables of U / U2 */
+int e2d_conserved_sums(e2d_run *run, int which, double sums[4]);
+/* accumulated device time in seconds of the five timers of HydroRun.h:74-75:
+ * boundaries, godunov, primitives, fluxes, update (the fused schedule books
+ * its whole step under godunov) */
+int e2d_timers(e2d_run *run, double out[5]);
+int e2d_set_schedule(e2d_run *run, int schedule);
+int e2d_synchronize(e2d_run *run);
+/* Device-side timing of `nsteps` loop iterations (e2d_run_loop semantics) with
+ * HIP events on the handle's stream; returns the elapsed milliseconds and the
+ * number of steps actually executed.  Used by benchmarks. */
+int e2d_time_loop(e2d_run *run, int nsteps, double *elapsed_ms, int *executed, double *t,
+                  int *nStep);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EULER2D_HIP_H_ */

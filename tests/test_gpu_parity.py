@@ -1,0 +1,139 @@
+"""FP64 HIP path against the CPU oracle on the same seeded inputs.
+
+Gate: relative L1 <= 1e-12 per conserved variable after N
+steps.  The FP64 kernels are built to reproduce the oracle bit for bit, so the
+tests also check exact equality of the full arrays (ghost cells included) and
+of the accumulated time t.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from euler2d_kokkos_amd import (E2D_LAYOUT_INTERIOR, E2D_LAYOUT_LEFT, E2D_SCHEDULE_FUSED,
+                                E2D_SCHEDULE_PASSES, E2D_U, E2D_U2, HydroParams, HydroRun)
+
+pytestmark = pytest.mark.gpu
+
+RTOL_L1 = 1e-12
+
+CASES = {
+    "implode64": dict(nx=64, ny=64, problemType=0, cfl=0.8, nStepmax=60, tEnd=1e9),
+    "implode_aniso": dict(nx=70, ny=45, problemType=0, cfl=0.8, nStepmax=40, tEnd=1e9,
+                          xmin=-1.0, xmax=1.0, ymax=1.0, slope_type=2.0, gamma0=1.666),
+    "blast48x72": dict(nx=48, ny=72, problemType=1, cfl=0.7, nStepmax=60, tEnd=1e9, ymax=1.5,
+                       blast_radius=0.1, blast_center_x=0.5, blast_center_y=0.75,
+                       blast_density_in=1.0, blast_density_out=1.0, blast_pressure_in=10.0,
+                       blast_pressure_out=0.1),
+    "four_quadrant64_bc2": dict(nx=64, ny=64, problemType=2, cfl=0.8, nStepmax=50, tEnd=1e9,
+                                boundary_type_xmin=2, boundary_type_xmax=2,
+                                boundary_type_ymin=2, boundary_type_ymax=2),
+    "shocked_bubble96x48": dict(nx=96, ny=48, problemType=4, cfl=0.8, nStepmax=40, tEnd=1e9,
+                                xmax=2.0, boundary_type_xmin=2, boundary_type_xmax=2,
+                                sb_preshock_density=1.0, sb_preshock_pressure=1.0,
+                                sb_postshock_density=3.857142857142857,
+                                sb_postshock_pressure=10.333333333333332,
+                                sb_postshock_velocity=2.6293687924887177, sb_shock_loc=0.1,
+                                sb_bubble_density=0.1, sb_bubble_pressure=1.0,
+                                sb_bubble_radius=0.15, sb_bubble_center_x=0.4,
+                                sb_bubble_center_y=0.5),
+    "discontinuity64": dict(nx=64, ny=64, problemType=3, cfl=0.8, nStepmax=50, tEnd=1e9),
+    "periodic_blast": dict(nx=61, ny=67, problemType=1, cfl=0.6, nStepmax=40, tEnd=1e9,
+                           boundary_type_xmin=3, boundary_type_xmax=3, boundary_type_ymin=3,
+                           boundary_type_ymax=3, blast_radius=0.2, blast_center_x=0.3,
+                           blast_center_y=0.6),
+    "mixed_bc_slope0": dict(nx=33, ny=129, problemType=1, cfl=0.5, nStepmax=30, tEnd=1e9,
+                            boundary_type_xmin=1, boundary_type_xmax=2, boundary_type_ymin=3,
+                            boundary_type_ymax=3, slope_type=0.0, blast_radius=0.25),
+    "tend_clamp": dict(nx=40, ny=40, problemType=0, cfl=0.8, nStepmax=1000, tEnd=0.05),
+    "sedov_energy": dict(nx=50, ny=50, problemType=1, cfl=0.7, nStepmax=30, tEnd=1e9,
+                         blast_total_energy_inside=1.0, blast_radius=0.05,
+                         blast_density_out=1.0),
+}
+
+
+def _params(case):
+    return O.default_params(**CASES[case])
+
+
+def _gpu_run(p, schedule, steps=0):
+    hp = HydroParams(p.copy())
+    run = HydroRun(hp, schedule=schedule)
+    run.run(steps)
+    return run
+
+
+def _compare(run, ora, tag):
+    assert run.nStep == ora.nStep, (tag, run.nStep, ora.nStep)
+    assert run.t == ora.t, (tag, run.t, ora.t)
+    g = run.get_state(run.current())
+    c = ora.current()
+    err = O.rel_l1(O.interior(g), O.interior(c))
+    print(f"{tag}: steps={run.nStep} t={run.t!r} relL1={err} bitwise={np.array_equal(g, c)}")
+    assert np.all(err <= RTOL_L1), err
+    # both arrays, ghosts included, equal the oracle's
+    for which in (E2D_U, E2D_U2):
+        np.testing.assert_array_equal(run.get_state(which), ora.get(which), err_msg=tag)
+
+
+@pytest.mark.parametrize("schedule", [E2D_SCHEDULE_FUSED, E2D_SCHEDULE_PASSES],
+                         ids=["fused", "passes"])
+@pytest.mark.parametrize("case", list(CASES))
+def test_loop_matches_oracle(gpu_available, case, schedule):
+    p = _params(case)
+    ora = O.OracleRun(p)
+    ora.run()
+    run = _gpu_run(p, schedule)
+    _compare(run, ora, f"{case}/{schedule}")
+
+
+@pytest.mark.parametrize("case", ["implode64", "blast48x72"])
+def test_reference_api_steps(gpu_available, case):
+    """compute_dt / godunov_unsplit / make_boundaries called step by step."""
+    p = _params(case)
+    ora = O.OracleRun(p)
+    hp = HydroParams(p.copy())
+    run = HydroRun(hp)
+    assert run.compute_dt(0) == ora.compute_dt(0)
+    for which in (0, 1):
+        ora.make_boundaries(which)
+        run.make_boundaries(which)
+        np.testing.assert_array_equal(run.get_state(which), ora.get(which))
+    t = 0.0
+    for n in range(12):
+        dt_o = ora.compute_dt(n % 2)
+        dt_g = run.compute_dt(n % 2)
+        assert dt_g == dt_o, (n, dt_g, dt_o)
+        ora.godunov_unsplit(n, dt_o)
+        run.godunov_unsplit(n, dt_g)
+        t += dt_o
+    for which in (0, 1):
+        np.testing.assert_array_equal(run.get_state(which), ora.get(which))
+
+
+def test_layouts_roundtrip(gpu_available):
+    p = _params("implode_aniso")
+    run = HydroRun(HydroParams(p.copy()))
+    rng = np.random.default_rng(7)
+    a = rng.random((p.isize, p.jsize, 4)) + 0.5
+    run.set_state(E2D_U, a)
+    np.testing.assert_array_equal(run.get_state(E2D_U), a)
+    left = run.get_state(E2D_U, E2D_LAYOUT_LEFT)
+    np.testing.assert_array_equal(left, np.transpose(a, (2, 1, 0)))
+    inter = run.get_state(E2D_U, E2D_LAYOUT_INTERIOR)
+    np.testing.assert_array_equal(inter, np.transpose(a[2:-2, 2:-2, :], (2, 1, 0)))
+    b = rng.random(inter.shape)
+    run.set_state(E2D_U2, b, E2D_LAYOUT_INTERIOR)
+    np.testing.assert_array_equal(run.get_state(E2D_U2, E2D_LAYOUT_INTERIOR), b)
+
+
+def test_conserved_sums_reflecting(gpu_available):
+    """Reflecting walls: mass and energy conserved to summation roundoff."""
+    p = _params("blast48x72")
+    run = HydroRun(HydroParams(p.copy()))
+    s0 = run.conserved_sums(E2D_U)
+    ref0 = O.interior(run.get_state(E2D_U)).reshape(-1, 4).sum(0)
+    np.testing.assert_allclose(s0, ref0, rtol=1e-14)
+    run.run(200)
+    s1 = run.conserved_sums(run.current())
+    assert abs(s1[0] - s0[0]) <= 1e-12 * abs(s0[0])
+    assert abs(s1[1] - s0[1]) <= 1e-12 * abs(s0[1])
