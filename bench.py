@@ -1,0 +1,135 @@
+"""Benchmark of the unsplit MUSCL-Hancock Godunov timestep on one MI355X.
+
+Workload: the blast problem on 1024x1536 cells ([0,1]x[0,1.5], gamma 1.4,
+p=10 in a disk of radius 0.1 at (0.5,0.75), p=0.1 outside, reflecting walls,
+HLLC, CFL 0.7), FP64, 2000 steps from t=0 -- one step is one timestep of the
+whole grid (dt reduction included, dt kept on the device).  The state lives in
+HBM before the timed region starts; nothing crosses PCIe inside it.
+
+  python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+
+Prints one JSON line.  `value` is interior cell-updates per second over the
+timed region (host clock, stream synchronised on both sides).  `roofline` uses
+the step kernel's average duration from HIP events on the solver's stream and
+the algorithmic 64 bytes per cell-update (read 4 doubles, write 4 doubles).
+`cpu_baseline` times the CPU oracle (a restatement of the reference's
+implementation 0 with OpenMP) on this host over a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mcell-updates/s on 1 MI355X: FP64 plain vs FP32/batched; % of HBM roofline"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = 64.0
+DECK = ROOT / "euler2d_kokkos_amd" / "decks" / "blast_1024x1536.ini"
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="bound on the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    return ap.parse_args()
+
+
+def cpu_baseline(deck: Path, seconds: float) -> dict:
+    """Times the CPU oracle on this host over a bounded sample of the workload."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O  # test infrastructure, used here only as the baseline
+
+    threads = int(os.environ.get("E2D_CPU_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = min(threads, 16)
+    O.lib().ora_set_threads(threads)
+    p = O.params_from_ini(deck)
+    run = O.OracleRun(p)
+    run.run(1)  # warm-up step (first touch of the arrays)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and run.nStep < p.nStepmax:
+        run.run(1)
+        steps += 1
+    el = time.perf_counter() - t0
+    value = p.nx * p.ny * steps / el / 1e6
+    return {"value": round(value, 3), "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"blast 1024x1536 FP64, {steps} steps after 1 warm-up step "
+                      f"(of 2000), CPU oracle with OpenMP over {threads} threads, "
+                      f"{el:.1f} s"}
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU "
+                          "domain decomposition is a design note, not a code path",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    from euler2d_kokkos_amd import E2D_SCHEDULE_FUSED, HydroParams, HydroRun, device_info
+
+    dev, ncu = device_info()
+    params = HydroParams.from_ini(DECK)
+    params.nStepmax = max(params.nStepmax, args.steps)
+    run = HydroRun(params, schedule=E2D_SCHEDULE_FUSED)
+
+    if args.warmup > 0:
+        run.run(args.warmup)
+        run.synchronize()
+    run.init_problem()  # the timed run starts from t = 0 like the workload
+    run.synchronize()
+
+    t0 = time.perf_counter()
+    kernel_ms, executed = run.time_loop(args.steps)
+    run.synchronize()
+    wall = time.perf_counter() - t0
+    if executed != args.steps:
+        raise RuntimeError(f"ran {executed} steps instead of {args.steps}")
+
+    cells = params.nx * params.ny
+    value = cells * args.steps / wall / 1e6
+    per_step_ms = kernel_ms / args.steps
+    achieved = BYTES_PER_CELL * cells / (per_step_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (blast initial condition generated on the GPU)",
+        "config": {"workload": "blast 1024x1536 FP64 HLLC CFL 0.7 reflecting walls, "
+                               "2000 steps from t=0", "nx": params.nx, "ny": params.ny,
+                   "precision": "fp64", "schedule": "fused", "device": dev, "cus": ncu},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "kernel_ms_per_step": round(per_step_ms, 5),
+        "final_t": run.t,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(DECK, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

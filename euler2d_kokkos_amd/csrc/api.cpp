@@ -246,8 +246,9 @@ int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elap
   e2d::StepCtl c{};
   int launched = 0;
   for (;;) {
-    int todo = kChunk;
-    if (max_steps > 0) todo = std::min(todo, max_steps - launched);
+    // with an explicit step budget every launch is queued at once (steps past
+    // the end of the loop return immediately); otherwise check every kChunk
+    int todo = max_steps > 0 ? max_steps - launched : kChunk;
     const long long left = (long long)r->p.nStepmax - (n0 + launched);
     if (left < todo) todo = (int)std::max(0LL, left);
     if (todo <= 0) break;
@@ -629,6 +630,19 @@ int e2d_set_schedule(e2d_run *r, int schedule) {
   if (schedule != E2D_SCHEDULE_FUSED && schedule != E2D_SCHEDULE_PASSES)
     return fail(E2D_ERR_ARG, "unknown schedule");
   r->schedule = schedule;
+  return E2D_OK;
+}
+
+int e2d_set_tile(e2d_run *r, int seg_rows) {
+  if (!r) return fail(E2D_ERR_ARG, "null e2d_run handle");
+  if (seg_rows < 1 || seg_rows > 4096) return fail(E2D_ERR_ARG, "seg_rows must be in [1, 4096]");
+  r->seg_rows = seg_rows;
+  return E2D_OK;
+}
+
+int e2d_get_tile(e2d_run *r, int *seg_rows) {
+  if (!r || !seg_rows) return fail(E2D_ERR_ARG, "null argument");
+  *seg_rows = r->seg_rows;
   return E2D_OK;
 }
 

@@ -236,6 +236,7 @@ __device__ __forceinline__ void slopes(const Scheme &s, const Q4 &q, const Q4 &p
 }
 
 // ComputeAndStoreFluxesFunctor (:449-644), i,j in [2, size-2]
+template <int RS>
 __global__ void fluxes_kernel(KParams kp, Grid Q, Grid Fx, Grid Fy, double dtdx, double dtdy) {
   const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -252,7 +253,7 @@ __global__ void fluxes_kernel(KParams kp, Grid Q, Grid Fx, Grid Fy, double dtdx,
     const Sources sn = trace_sources(s, qn, dqXn, dqYn);
     trace_face(s, qc, dqX, sc, dtdx, false, fr);
     trace_face(s, qn, dqXn, sn, dtdx, true, fl);
-    riemann(s, fl, fr, IU, IV, flux);
+    riemann<RS>(s, fl, fr, IU, IV, flux);
     for (int v = 0; v < 4; v++) at(Fx, v, i, j) = flux[v] * dtdx;
   }
   {
@@ -261,7 +262,7 @@ __global__ void fluxes_kernel(KParams kp, Grid Q, Grid Fx, Grid Fy, double dtdx,
     const Sources sn = trace_sources(s, qn, dqXn, dqYn);
     trace_face(s, qc, dqY, sc, dtdy, false, fr);
     trace_face(s, qn, dqYn, sn, dtdy, true, fl);
-    riemann(s, fl, fr, IV, IU, flux);
+    riemann<RS>(s, fl, fr, IV, IU, flux);
     for (int v = 0; v < 4; v++) at(Fy, v, i, j) = flux[v] * dtdy;
   }
 }
@@ -312,6 +313,7 @@ __device__ __forceinline__ Q4 shfl_dn1(const Q4 &a) {
   return r;
 }
 
+template <int RS>
 __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in, Grid out,
                                                             double dt_arg, StepCtl *ctl,
                                                             int seg_rows, int nstrips, int nwaves) {
@@ -397,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
         if (k >= 3) {
           // y face between rows a-1 and a
           double f[4];
-          riemann(s, fymax_c, fymin, IV, IU, f);
+          riemann<RS>(s, fymax_c, fymin, IV, IU, f);
           Q4 fya;
           for (int v = 0; v < 4; v++) fya.v[v] = f[v] * dtdy;
           Q4 fxa = {};
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
             trace_face(s, qm1, dqX, src, dtdx, false, fxmin);
             trace_face(s, qm1, dqX, src, dtdx, true, fxmax);
             const Q4 ql = shfl_up1(fxmax);
-            riemann(s, ql, fxmin, IU, IV, f);
+            riemann<RS>(s, ql, fxmin, IU, IV, f);
             for (int v = 0; v < 4; v++) fxa.v[v] = f[v] * dtdx;
           }
           if (k >= 4) {
@@ -582,8 +584,13 @@ hipError_t launch_prim(const KParams &kp, Grid U, Grid Q, hipStream_t st) {
 hipError_t launch_fluxes(const KParams &kp, Grid Q, Grid Fx, Grid Fy, double dtdx, double dtdy,
                          hipStream_t st) {
   const dim3 blk(64, 4);
-  hipLaunchKernelGGL(fluxes_kernel, grid2d(kp.nx + 1, kp.ny + 1, blk), blk, 0, st, kp, Q, Fx, Fy,
-                     dtdx, dtdy);
+  const dim3 grd = grid2d(kp.nx + 1, kp.ny + 1, blk);
+  if (kp.s.riemann == 2)
+    hipLaunchKernelGGL(fluxes_kernel<2>, grd, blk, 0, st, kp, Q, Fx, Fy, dtdx, dtdy);
+  else if (kp.s.riemann == 1)
+    hipLaunchKernelGGL(fluxes_kernel<1>, grd, blk, 0, st, kp, Q, Fx, Fy, dtdx, dtdy);
+  else
+    hipLaunchKernelGGL(fluxes_kernel<0>, grd, blk, 0, st, kp, Q, Fx, Fy, dtdx, dtdy);
   return hipGetLastError();
 }
 
@@ -601,8 +608,15 @@ hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, St
   const int nwaves = nstrips * nsegs;
   const int waves_per_block = kBlock / kLanes;
   const int blocks = (nwaves + waves_per_block - 1) / waves_per_block;
-  hipLaunchKernelGGL(step_fused_kernel, dim3(blocks), dim3(kBlock), 0, st, kp, in, out, dt, ctl,
-                     seg_rows, nstrips, nwaves);
+  if (kp.s.riemann == 2)
+    hipLaunchKernelGGL(step_fused_kernel<2>, dim3(blocks), dim3(kBlock), 0, st, kp, in, out, dt,
+                       ctl, seg_rows, nstrips, nwaves);
+  else if (kp.s.riemann == 1)
+    hipLaunchKernelGGL(step_fused_kernel<1>, dim3(blocks), dim3(kBlock), 0, st, kp, in, out, dt,
+                       ctl, seg_rows, nstrips, nwaves);
+  else
+    hipLaunchKernelGGL(step_fused_kernel<0>, dim3(blocks), dim3(kBlock), 0, st, kp, in, out, dt,
+                       ctl, seg_rows, nstrips, nwaves);
   return hipGetLastError();
 }
 

@@ -279,11 +279,13 @@ __device__ __forceinline__ void approx(const Scheme &s, const Q4 &qlf, const Q4 
   flux[IP] = gu * (etot + gp);
 }
 
+// solver chosen at compile time: RS = 0 approx, 1 hll, 2 hllc
+template <int RS>
 __device__ __forceinline__ void riemann(const Scheme &s, const Q4 &ql, const Q4 &qr, int ln, int tn,
                                         double flux[4]) {
-  if (s.riemann == 2)
+  if constexpr (RS == 2)
     hllc(s, ql, qr, ln, tn, flux);
-  else if (s.riemann == 1)
+  else if constexpr (RS == 1)
     hll(s, ql, qr, ln, tn, flux);
   else
     approx(s, ql, qr, ln, tn, flux);

@@ -183,6 +183,14 @@ class HydroRun:
         self.t, self.nStep = t.value, n.value
         return ms.value, done.value
 
+    def set_tile(self, seg_rows: int) -> None:
+        check(load_library().e2d_set_tile(self._h, seg_rows))
+
+    def get_tile(self) -> int:
+        out = C.c_int(0)
+        check(load_library().e2d_get_tile(self._h, C.byref(out)))
+        return out.value
+
     def synchronize(self) -> None:
         check(load_library().e2d_synchronize(self._h))
 

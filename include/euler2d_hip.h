@@ -181,6 +181,10 @@ int e2d_conserved_sums(e2d_run *run, int which, double sums[4]);
  * its whole step under godunov) */
 int e2d_timers(e2d_run *run, double out[5]);
 int e2d_set_schedule(e2d_run *run, int schedule);
+/* rows marched by one wavefront of the fused step kernel (1..4096); the
+ * wavefront covers 60 output columns.  Results do not depend on it. */
+int e2d_set_tile(e2d_run *run, int seg_rows);
+int e2d_get_tile(e2d_run *run, int *seg_rows);
 int e2d_synchronize(e2d_run *run);
 /* Device-side timing of `nsteps` loop iterations (e2d_run_loop semantics) with
  * HIP events on the handle's stream; returns the elapsed milliseconds and the
