@@ -41,6 +41,9 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="K",
+                    help="only launch the plain whole-array copy kernel K times (the known-size "
+                         "reference for rocprof FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
 
 
@@ -83,6 +86,18 @@ def main() -> int:
     params = HydroParams.from_ini(DECK)
     params.nStepmax = max(params.nStepmax, args.steps)
     run = HydroRun(params, schedule=E2D_SCHEDULE_FUSED)
+
+    if args.calibrate_copy > 0:
+        from euler2d_kokkos_amd import E2D_U, E2D_U2
+        pitch = (params.isize + 15) // 16 * 16
+        plane = (pitch * params.jsize + 63) // 64 * 64
+        for _ in range(args.calibrate_copy):
+            run.copy_array(E2D_U, E2D_U2)
+        run.synchronize()
+        print(json.dumps({"copy_kernel_launches": args.calibrate_copy,
+                          "bytes_read_per_launch": plane * 4 * 8,
+                          "bytes_written_per_launch": plane * 4 * 8}), flush=True)
+        return 0
 
     if args.warmup > 0:
         run.run(args.warmup)

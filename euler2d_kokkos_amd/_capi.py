@@ -107,6 +107,7 @@ _PROTOTYPES = {
     "e2d_run_loop": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "e2d_copy_state": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "e2d_upload_state": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "e2d_copy_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "e2d_conserved_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "e2d_timers": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "e2d_set_schedule": (C.c_int, [C.c_void_p, C.c_int]),

@@ -127,8 +127,7 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
     if (rc) return rc;
     if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
     HIP_TRY(e2d::launch_boundaries(r->kp, in, r->stream));
-    HIP_TRY(hipMemcpyAsync(out.p, in.p, (size_t)in.plane * 4 * sizeof(double),
-                           hipMemcpyDeviceToDevice, r->stream));
+    HIP_TRY(e2d::launch_copy(in, out, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[1], r->stream));
     HIP_TRY(e2d::launch_prim(r->kp, in, r->Q, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[2], r->stream));
@@ -482,9 +481,18 @@ int e2d_init_problem(e2d_run *r) {
     HIP_TRY(e2d::launch_blast_energy(r->p, r->U, r->p.blast_total_energy_inside / volume,
                                      r->stream));
   }
-  HIP_TRY(hipMemcpyAsync(r->U2.p, r->U.p, (size_t)r->U.plane * 4 * sizeof(double),
-                         hipMemcpyDeviceToDevice, r->stream));
+  HIP_TRY(e2d::launch_copy(r->U, r->U2, r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
+  return E2D_OK;
+}
+
+int e2d_copy_array(e2d_run *r, int src, int dst) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if ((src != E2D_U && src != E2D_U2) || (dst != E2D_U && dst != E2D_U2))
+    return fail(E2D_ERR_ARG, "src/dst must be E2D_U or E2D_U2");
+  if (src == dst) return E2D_OK;
+  HIP_TRY(e2d::launch_copy(state(r, src), state(r, dst), r->stream));
   return E2D_OK;
 }
 

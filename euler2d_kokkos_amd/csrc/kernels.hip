@@ -507,6 +507,15 @@ __global__ void sums_final_kernel(const double *partials, int nblocks, double *o
   if (threadIdx.x < 4) out4[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Plain device copy (Kokkos::deep_copy of a whole state array, HydroRun.h:214),
+// 16 bytes per lane per access.
+__global__ void copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                            long long n2) {
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n2;
+       k += (long long)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
 dim3 grid2d(int nx, int ny, dim3 blk) {
   return dim3((unsigned)((nx + blk.x - 1) / blk.x), (unsigned)((ny + blk.y - 1) / blk.y));
 }
@@ -621,6 +630,16 @@ hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, St
 }
 
 int sums_partials_needed(const KParams &) { return kSumBlocks * 4; }
+
+hipError_t launch_copy(Grid src, Grid dst, hipStream_t st) {
+  const long long n2 = src.plane * 4 / 2;  // planes are 64-element aligned
+  long long blocks = (n2 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
+                     reinterpret_cast<const double2 *>(src.p), reinterpret_cast<double2 *>(dst.p),
+                     n2);
+  return hipGetLastError();
+}
 
 hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st) {
   hipLaunchKernelGGL(sums_partial_kernel, dim3(kSumBlocks), dim3(256), 0, st, kp, U, partials);

@@ -62,5 +62,7 @@ hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, St
                              int seg_rows, hipStream_t st);
 hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st);
 int sums_partials_needed(const KParams &kp);
+// whole-array copy (all four planes, ghosts and padding included)
+hipError_t launch_copy(Grid src, Grid dst, hipStream_t st);
 
 }  // namespace e2d

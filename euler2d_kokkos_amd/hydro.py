@@ -227,6 +227,9 @@ class HydroRun:
         """Selector of the state after nStep steps (main.cpp:155-158)."""
         return _capi.E2D_U if self.nStep % 2 == 0 else _capi.E2D_U2
 
+    def copy_array(self, src: int, dst: int) -> None:
+        check(load_library().e2d_copy_array(self._h, src, dst))
+
     def conserved_sums(self, which: int) -> np.ndarray:
         out = (C.c_double * 4)()
         check(load_library().e2d_conserved_sums(self._h, which, out))

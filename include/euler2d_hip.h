@@ -174,6 +174,9 @@ int e2d_run_loop(e2d_run *run, int max_steps, double *t, int *nStep);
 /* device <-> host copies of U / U2 in one of the E2D_LAYOUT_* layouts */
 int e2d_copy_state(e2d_run *run, int which, double *host, int layout);
 int e2d_upload_state(e2d_run *run, int which, const double *host, int layout);
+/* Kokkos::deep_copy(dst, src) of a whole state array (HydroRun.h:214,302),
+ * asynchronous on the handle's stream */
+int e2d_copy_array(e2d_run *run, int src, int dst);
 /* FP64 interior sums of the four conserved variables of U / U2 */
 int e2d_conserved_sums(e2d_run *run, int which, double sums[4]);
 /* accumulated device time in seconds of the five timers of HydroRun.h:74-75:
