@@ -78,14 +78,17 @@ int alloc_dev(e2d_run *r, void **ptr, size_t bytes) {
   return E2D_OK;
 }
 
-int alloc_grid(e2d_run *r, Grid &g) {
-  const int pitch = (r->p.isize + 15) / 16 * 16;  // rows start on 128-byte boundaries
+// Rows start on 128-byte and planes on 512-byte boundaries.
+int alloc_grid(e2d_run *r, Grid &g, int precision) {
+  const int sb = e2d::grid_scalar_bytes(precision);
+  const int row_align = 128 / sb, plane_align = 512 / sb;
+  const int pitch = (r->p.isize + row_align - 1) / row_align * row_align;
   long long plane = (long long)pitch * r->p.jsize;
-  plane = (plane + 63) / 64 * 64;  // planes start on 512-byte boundaries
+  plane = (plane + plane_align - 1) / plane_align * plane_align;
   void *ptr = nullptr;
-  int rc = alloc_dev(r, &ptr, (size_t)plane * 4 * sizeof(double));
+  int rc = alloc_dev(r, &ptr, (size_t)plane * e2d::grid_planes(precision) * sb);
   if (rc) return rc;
-  g.p = static_cast<double *>(ptr);
+  g.p = ptr;
   g.pitch = pitch;
   g.plane = plane;
   return E2D_OK;
@@ -102,9 +105,9 @@ int check_run(e2d_run *r) {
 
 int ensure_pass_arrays(e2d_run *r) {
   if (r->Q.p) return E2D_OK;
-  int rc = alloc_grid(r, r->Q);
-  if (!rc) rc = alloc_grid(r, r->Fx);
-  if (!rc) rc = alloc_grid(r, r->Fy);
+  int rc = alloc_grid(r, r->Q, E2D_PRECISION_FP64);
+  if (!rc) rc = alloc_grid(r, r->Fx, E2D_PRECISION_FP64);
+  if (!rc) rc = alloc_grid(r, r->Fy, E2D_PRECISION_FP64);
   if (!rc) {
     const size_t bytes = (size_t)r->Fx.plane * 4 * sizeof(double);
     HIP_TRY(hipMemsetAsync(r->Fx.p, 0, bytes, r->stream));
@@ -123,11 +126,13 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
   if (timed)
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
   if (r->schedule == E2D_SCHEDULE_PASSES) {
+    if (r->precision != E2D_PRECISION_FP64)
+      return fail(E2D_ERR_UNSUPPORTED, "the pass schedule exists for FP64 only");
     int rc = ensure_pass_arrays(r);
     if (rc) return rc;
     if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
     HIP_TRY(e2d::launch_boundaries(r->kp, in, r->stream));
-    HIP_TRY(e2d::launch_copy(in, out, r->stream));
+    HIP_TRY(e2d::launch_copy(r->kp, in, out, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[1], r->stream));
     HIP_TRY(e2d::launch_prim(r->kp, in, r->Q, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[2], r->stream));
@@ -273,6 +278,100 @@ int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elap
   return E2D_OK;
 }
 
+// Device state -> host FP64 planes [v][j][i] (isize x jsize, ghosts included);
+// the split representation is combined as (double)hi + (double)lo.
+int download_planes(e2d_run *r, const Grid &g, std::vector<double> &out) {
+  const int isz = r->p.isize, jsz = r->p.jsize;
+  const size_t cells = (size_t)isz * jsz;
+  out.assign(4 * cells, 0.0);
+  if (r->precision == E2D_PRECISION_FP64) {
+    const double *src = static_cast<const double *>(g.p);
+    for (int v = 0; v < 4; v++)
+      HIP_TRY(hipMemcpy2DAsync(out.data() + v * cells, (size_t)isz * sizeof(double),
+                               src + v * g.plane, (size_t)g.pitch * sizeof(double),
+                               (size_t)isz * sizeof(double), (size_t)jsz, hipMemcpyDeviceToHost,
+                               r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return E2D_OK;
+  }
+  std::vector<float> tmp(8 * cells);
+  const float *src = static_cast<const float *>(g.p);
+  for (int k = 0; k < 8; k++)
+    HIP_TRY(hipMemcpy2DAsync(tmp.data() + k * cells, (size_t)isz * sizeof(float),
+                             src + k * g.plane, (size_t)g.pitch * sizeof(float),
+                             (size_t)isz * sizeof(float), (size_t)jsz, hipMemcpyDeviceToHost,
+                             r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  for (int v = 0; v < 4; v++)
+    for (size_t c = 0; c < cells; c++)
+      out[v * cells + c] = (double)tmp[(2 * v) * cells + c] + (double)tmp[(2 * v + 1) * cells + c];
+  return E2D_OK;
+}
+
+// Host FP64 planes -> device state (split: hi = (float)x, lo = (float)(x - hi)).
+int upload_planes(e2d_run *r, const Grid &g, const std::vector<double> &in) {
+  const int isz = r->p.isize, jsz = r->p.jsize;
+  const size_t cells = (size_t)isz * jsz;
+  if (r->precision == E2D_PRECISION_FP64) {
+    double *dst = static_cast<double *>(g.p);
+    for (int v = 0; v < 4; v++)
+      HIP_TRY(hipMemcpy2DAsync(dst + v * g.plane, (size_t)g.pitch * sizeof(double),
+                               in.data() + v * cells, (size_t)isz * sizeof(double),
+                               (size_t)isz * sizeof(double), (size_t)jsz, hipMemcpyHostToDevice,
+                               r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return E2D_OK;
+  }
+  std::vector<float> tmp(8 * cells);
+  for (int v = 0; v < 4; v++)
+    for (size_t c = 0; c < cells; c++) {
+      const double x = in[v * cells + c];
+      const float hi = (float)x;
+      tmp[(2 * v) * cells + c] = hi;
+      tmp[(2 * v + 1) * cells + c] = (float)(x - (double)hi);
+    }
+  float *dst = static_cast<float *>(g.p);
+  for (int k = 0; k < 8; k++)
+    HIP_TRY(hipMemcpy2DAsync(dst + k * g.plane, (size_t)g.pitch * sizeof(float),
+                             tmp.data() + k * cells, (size_t)isz * sizeof(float),
+                             (size_t)isz * sizeof(float), (size_t)jsz, hipMemcpyHostToDevice,
+                             r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return E2D_OK;
+}
+
+void planes_to_layout(const e2d_run *r, const std::vector<double> &pl, double *host, int layout) {
+  const int isz = r->p.isize, jsz = r->p.jsize, nx = r->p.nx, ny = r->p.ny;
+  const size_t cells = (size_t)isz * jsz;
+  for (int v = 0; v < 4; v++)
+    for (int j = 0; j < jsz; j++)
+      for (int i = 0; i < isz; i++) {
+        const double x = pl[v * cells + (size_t)j * isz + i];
+        if (layout == E2D_LAYOUT_RIGHT)
+          host[((size_t)i * jsz + j) * 4 + v] = x;
+        else if (layout == E2D_LAYOUT_LEFT)
+          host[v * cells + (size_t)j * isz + i] = x;
+        else if (i >= 2 && i < nx + 2 && j >= 2 && j < ny + 2)
+          host[(size_t)v * nx * ny + (size_t)(j - 2) * nx + (i - 2)] = x;
+      }
+}
+
+void layout_to_planes(const e2d_run *r, const double *host, int layout, std::vector<double> &pl) {
+  const int isz = r->p.isize, jsz = r->p.jsize, nx = r->p.nx, ny = r->p.ny;
+  const size_t cells = (size_t)isz * jsz;
+  for (int v = 0; v < 4; v++)
+    for (int j = 0; j < jsz; j++)
+      for (int i = 0; i < isz; i++) {
+        double &x = pl[v * cells + (size_t)j * isz + i];
+        if (layout == E2D_LAYOUT_RIGHT)
+          x = host[((size_t)i * jsz + j) * 4 + v];
+        else if (layout == E2D_LAYOUT_LEFT)
+          x = host[v * cells + (size_t)j * isz + i];
+        else if (i >= 2 && i < nx + 2 && j >= 2 && j < ny + 2)
+          x = host[(size_t)v * nx * ny + (size_t)(j - 2) * nx + (i - 2)];
+      }
+}
+
 template <class F>
 int guarded(F &&f) {
   try {
@@ -410,15 +509,15 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
   *out = nullptr;
   int rc = e2d_params_validate(p);
   if (rc) return rc;
-  if (precision != E2D_PRECISION_FP64)
-    return fail(E2D_ERR_UNSUPPORTED, "precision not available in this build");
+  if (precision != E2D_PRECISION_FP64 && precision != E2D_PRECISION_FP32_SPLIT)
+    return fail(E2D_ERR_ARG, "precision must be E2D_PRECISION_FP64 or E2D_PRECISION_FP32_SPLIT");
   return guarded([&]() -> int {
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) return fail(E2D_ERR_HIP, "no HIP device visible");
     auto *r = new e2d_run();
     r->p = *p;
-    r->kp = e2d::make_kparams(*p);
+    r->kp = e2d::make_kparams(*p, precision);
     r->precision = precision;
     HIP_TRY(hipGetDevice(&r->device));
     e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
@@ -427,8 +526,8 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
       return hip_fail(e, "hipStreamCreate");
     }
     void *ptr = nullptr;
-    int rc2 = alloc_grid(r, r->U);
-    if (!rc2) rc2 = alloc_grid(r, r->U2);
+    int rc2 = alloc_grid(r, r->U, precision);
+    if (!rc2) rc2 = alloc_grid(r, r->U2, precision);
     if (!rc2) rc2 = alloc_dev(r, &ptr, sizeof(e2d::StepCtl));
     if (!rc2) r->ctl = static_cast<e2d::StepCtl *>(ptr);
     if (!rc2) rc2 = alloc_dev(r, &ptr, 64);
@@ -445,7 +544,8 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
       return fail(rc2, msg);
     }
     // defined contents everywhere (ghosts included) before the first init
-    const size_t bytes = (size_t)r->U.plane * 4 * sizeof(double);
+    const size_t bytes =
+      (size_t)r->U.plane * e2d::grid_planes(precision) * e2d::grid_scalar_bytes(precision);
     HIP_TRY(hipMemsetAsync(r->U.p, 0, bytes, r->stream));
     HIP_TRY(hipMemsetAsync(r->U2.p, 0, bytes, r->stream));
     HIP_TRY(hipMemsetAsync(r->ctl, 0, sizeof(e2d::StepCtl), r->stream));
@@ -470,7 +570,7 @@ int e2d_init_problem(e2d_run *r) {
   if (rc) return rc;
   const bool sedov = r->p.problemType == 1 && r->p.blast_total_energy_inside > 0;
   if (sedov) HIP_TRY(hipMemsetAsync(r->count, 0, sizeof(unsigned int), r->stream));
-  HIP_TRY(e2d::launch_init(r->p, r->U, sedov ? r->count : nullptr, r->stream));
+  HIP_TRY(e2d::launch_init(r->kp, r->p, r->U, sedov ? r->count : nullptr, r->stream));
   if (sedov) {
     // volume inside the disk, accumulated in a fixed sequential order
     unsigned int n = 0;
@@ -478,10 +578,10 @@ int e2d_init_problem(e2d_run *r) {
     HIP_TRY(hipStreamSynchronize(r->stream));
     double volume = 0.0;
     for (unsigned int k = 0; k < n; ++k) volume += r->p.dx * r->p.dy;
-    HIP_TRY(e2d::launch_blast_energy(r->p, r->U, r->p.blast_total_energy_inside / volume,
+    HIP_TRY(e2d::launch_blast_energy(r->kp, r->p, r->U, r->p.blast_total_energy_inside / volume,
                                      r->stream));
   }
-  HIP_TRY(e2d::launch_copy(r->U, r->U2, r->stream));
+  HIP_TRY(e2d::launch_copy(r->kp, r->U, r->U2, r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
   return E2D_OK;
 }
@@ -492,7 +592,7 @@ int e2d_copy_array(e2d_run *r, int src, int dst) {
   if ((src != E2D_U && src != E2D_U2) || (dst != E2D_U && dst != E2D_U2))
     return fail(E2D_ERR_ARG, "src/dst must be E2D_U or E2D_U2");
   if (src == dst) return E2D_OK;
-  HIP_TRY(e2d::launch_copy(state(r, src), state(r, dst), r->stream));
+  HIP_TRY(e2d::launch_copy(r->kp, state(r, src), state(r, dst), r->stream));
   return E2D_OK;
 }
 
@@ -544,37 +644,14 @@ int e2d_copy_state(e2d_run *r, int which, double *host, int layout) {
   int rc = check_run(r);
   if (rc) return rc;
   if (!host || (which != E2D_U && which != E2D_U2)) return fail(E2D_ERR_ARG, "bad argument");
+  if (layout != E2D_LAYOUT_INTERIOR && layout != E2D_LAYOUT_LEFT && layout != E2D_LAYOUT_RIGHT)
+    return fail(E2D_ERR_ARG, "unknown layout");
   const Grid &g = state(r, which);
-  const int isz = r->p.isize, jsz = r->p.jsize, nx = r->p.nx, ny = r->p.ny;
   return guarded([&]() -> int {
-    if (layout == E2D_LAYOUT_INTERIOR) {
-      for (int v = 0; v < 4; v++)
-        HIP_TRY(hipMemcpy2DAsync(host + (size_t)v * nx * ny, (size_t)nx * sizeof(double),
-                                 g.p + v * g.plane + 2LL * g.pitch + 2,
-                                 (size_t)g.pitch * sizeof(double), (size_t)nx * sizeof(double),
-                                 (size_t)ny, hipMemcpyDeviceToHost, r->stream));
-      HIP_TRY(hipStreamSynchronize(r->stream));
-      return E2D_OK;
-    }
-    if (layout != E2D_LAYOUT_LEFT && layout != E2D_LAYOUT_RIGHT)
-      return fail(E2D_ERR_ARG, "unknown layout");
-    std::vector<double> tmp;
-    double *dst = host;
-    if (layout == E2D_LAYOUT_RIGHT) {
-      tmp.resize((size_t)isz * jsz * 4);
-      dst = tmp.data();
-    }
-    for (int v = 0; v < 4; v++)
-      HIP_TRY(hipMemcpy2DAsync(dst + (size_t)v * isz * jsz, (size_t)isz * sizeof(double),
-                               g.p + v * g.plane, (size_t)g.pitch * sizeof(double),
-                               (size_t)isz * sizeof(double), (size_t)jsz, hipMemcpyDeviceToHost,
-                               r->stream));
-    HIP_TRY(hipStreamSynchronize(r->stream));
-    if (layout == E2D_LAYOUT_RIGHT)
-      for (int i = 0; i < isz; i++)
-        for (int j = 0; j < jsz; j++)
-          for (int v = 0; v < 4; v++)
-            host[((size_t)i * jsz + j) * 4 + v] = tmp[(size_t)v * isz * jsz + (size_t)j * isz + i];
+    std::vector<double> planes;
+    rc = download_planes(r, g, planes);
+    if (rc) return rc;
+    planes_to_layout(r, planes, host, layout);
     return E2D_OK;
   });
 }
@@ -583,37 +660,20 @@ int e2d_upload_state(e2d_run *r, int which, const double *host, int layout) {
   int rc = check_run(r);
   if (rc) return rc;
   if (!host || (which != E2D_U && which != E2D_U2)) return fail(E2D_ERR_ARG, "bad argument");
+  if (layout != E2D_LAYOUT_INTERIOR && layout != E2D_LAYOUT_LEFT && layout != E2D_LAYOUT_RIGHT)
+    return fail(E2D_ERR_ARG, "unknown layout");
   const Grid &g = state(r, which);
-  const int isz = r->p.isize, jsz = r->p.jsize, nx = r->p.nx, ny = r->p.ny;
   return guarded([&]() -> int {
+    std::vector<double> planes;
     if (layout == E2D_LAYOUT_INTERIOR) {
-      for (int v = 0; v < 4; v++)
-        HIP_TRY(hipMemcpy2DAsync(g.p + v * g.plane + 2LL * g.pitch + 2,
-                                 (size_t)g.pitch * sizeof(double), host + (size_t)v * nx * ny,
-                                 (size_t)nx * sizeof(double), (size_t)nx * sizeof(double),
-                                 (size_t)ny, hipMemcpyHostToDevice, r->stream));
-      HIP_TRY(hipStreamSynchronize(r->stream));
-      return E2D_OK;
+      // keep the ghost cells the device already holds
+      rc = download_planes(r, g, planes);
+      if (rc) return rc;
+    } else {
+      planes.assign((size_t)4 * r->p.isize * r->p.jsize, 0.0);
     }
-    if (layout != E2D_LAYOUT_LEFT && layout != E2D_LAYOUT_RIGHT)
-      return fail(E2D_ERR_ARG, "unknown layout");
-    std::vector<double> tmp;
-    const double *src = host;
-    if (layout == E2D_LAYOUT_RIGHT) {
-      tmp.resize((size_t)isz * jsz * 4);
-      for (int i = 0; i < isz; i++)
-        for (int j = 0; j < jsz; j++)
-          for (int v = 0; v < 4; v++)
-            tmp[(size_t)v * isz * jsz + (size_t)j * isz + i] = host[((size_t)i * jsz + j) * 4 + v];
-      src = tmp.data();
-    }
-    for (int v = 0; v < 4; v++)
-      HIP_TRY(hipMemcpy2DAsync(g.p + v * g.plane, (size_t)g.pitch * sizeof(double),
-                               src + (size_t)v * isz * jsz, (size_t)isz * sizeof(double),
-                               (size_t)isz * sizeof(double), (size_t)jsz, hipMemcpyHostToDevice,
-                               r->stream));
-    HIP_TRY(hipStreamSynchronize(r->stream));
-    return E2D_OK;
+    layout_to_planes(r, host, layout, planes);
+    return upload_planes(r, g, planes);
   });
 }
 

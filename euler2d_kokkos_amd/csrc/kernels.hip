@@ -1,11 +1,13 @@
-// kernels.hip -- FP64 HIP kernels for gfx950 (MI355X).
+// kernels.hip -- HIP kernels for gfx950 (MI355X).
 //
 // Reference passes reproduced (pkestene/euler2d_kokkos src/HydroRunFunctors.h):
 //   Init*Functor :1347-1827, MakeBoundariesFunctor :1832-2030, ComputeDtFunctor
 //   :17-79, ConvertToPrimitivesFunctor :84-143, ComputeAndStoreFluxesFunctor
 //   :412-651, UpdateFunctor :656-723.
 // step_fused_kernel performs BC -> prim -> slopes -> trace -> Riemann ->
-// update -> next-step dt in one pass over HBM, with bit-identical results.
+// update -> next-step dt in one pass over HBM.  Its FP64 instantiation gives
+// results bit-identical to the reference pass structure; its split-precision
+// instantiation computes in FP32 on a hi+lo FP32 conserved state.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -16,8 +18,134 @@ namespace e2d {
 
 namespace {
 
+// ----------------------------------------------------- storage policies
+// FP64: four double planes; the state is the value itself.
+struct P64 {
+  using T = double;
+  struct Cell {
+    double u[4];
+  };
+  static __device__ __forceinline__ Cell load(const Grid &g, long long off) {
+    const double *p = static_cast<const double *>(g.p);
+    Cell c;
+    for (int v = 0; v < 4; v++) c.u[v] = p[v * g.plane + off];
+    return c;
+  }
+  static __device__ __forceinline__ void store(const Grid &g, long long off, const Cell &c) {
+    double *p = static_cast<double *>(g.p);
+    for (int v = 0; v < 4; v++) p[v * g.plane + off] = c.u[v];
+  }
+  static __device__ __forceinline__ void negate(Cell &c, int v) { c.u[v] = -c.u[v]; }
+  static __device__ __forceinline__ Cell from_double(double d, double e, double mx, double my) {
+    return Cell{{d, e, mx, my}};
+  }
+  static __device__ __forceinline__ double value(const Cell &c, int v) { return c.u[v]; }
+  static __device__ __forceinline__ const SchemeT<T> &scheme(const KParams &kp) { return kp.s; }
+  static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
+    prim_no_c(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], q);
+  }
+  // UpdateFunctor order: (((U + Fx_i) - Fx_{i+1}) + Fy_j) - Fy_{j+1}
+  static __device__ __forceinline__ Cell update(const Cell &c, const Q4T<T> &fxl,
+                                                const Q4T<T> &fxr, const Q4T<T> &fyb,
+                                                const Q4T<T> &fyt) {
+    Cell o;
+    for (int v = 0; v < 4; v++) {
+      double u = c.u[v];
+      u += fxl.v[v];
+      u -= fxr.v[v];
+      u += fyb.v[v];
+      u -= fyt.v[v];
+      o.u[v] = u;
+    }
+    return o;
+  }
+};
+
+// Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no contraction).
+__device__ __forceinline__ float two_sum(float a, float b, float &e) {
+  const float s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+  return s;
+}
+
+// Split precision: hi and lo FP32 planes per variable.  Fluxes are FP32; each
+// one is added to the state with TwoSum so that the exact sum hi + lo changes
+// by exactly the flux (the same FP32 flux value leaves one cell and enters its
+// neighbour), which keeps the global sums conserved to FP32^2 rounding.
+struct PSplit {
+  using T = float;
+  struct Cell {
+    float hi[4], lo[4];
+  };
+  static __device__ __forceinline__ Cell load(const Grid &g, long long off) {
+    const float *p = static_cast<const float *>(g.p);
+    Cell c;
+    for (int v = 0; v < 4; v++) {
+      c.hi[v] = p[(2 * v) * g.plane + off];
+      c.lo[v] = p[(2 * v + 1) * g.plane + off];
+    }
+    return c;
+  }
+  static __device__ __forceinline__ void store(const Grid &g, long long off, const Cell &c) {
+    float *p = static_cast<float *>(g.p);
+    for (int v = 0; v < 4; v++) {
+      p[(2 * v) * g.plane + off] = c.hi[v];
+      p[(2 * v + 1) * g.plane + off] = c.lo[v];
+    }
+  }
+  static __device__ __forceinline__ void negate(Cell &c, int v) {
+    c.hi[v] = -c.hi[v];
+    c.lo[v] = -c.lo[v];
+  }
+  static __device__ __forceinline__ Cell from_double(double d, double e, double mx, double my) {
+    const double x[4] = {d, e, mx, my};
+    Cell c;
+    for (int v = 0; v < 4; v++) {
+      c.hi[v] = (float)x[v];
+      c.lo[v] = (float)(x[v] - (double)c.hi[v]);
+    }
+    return c;
+  }
+  static __device__ __forceinline__ double value(const Cell &c, int v) {
+    return (double)c.hi[v] + (double)c.lo[v];
+  }
+  static __device__ __forceinline__ const SchemeT<T> &scheme(const KParams &kp) { return kp.sf; }
+  // FP32 conversion; cells whose internal energy is a small fraction of the
+  // total energy (kinetic-dominated: p = (gamma-1)(E - rho v^2/2) cancels)
+  // are converted again in FP64 from hi + lo.
+  static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
+    prim_no_c(kp.sf, c.hi[0] + c.lo[0], c.hi[1] + c.lo[1], c.hi[2] + c.lo[2], c.hi[3] + c.lo[3],
+              q);
+    const float etot = c.hi[1];
+    if (!(q.v[IP] >= kp.kin_frac * (kp.sf.gamma0 - 1.0f) * etot)) {
+      Q4 q64;
+      prim_no_c(kp.s, value(c, 0), value(c, 1), value(c, 2), value(c, 3), q64);
+      for (int v = 0; v < 4; v++) q.v[v] = (float)q64.v[v];
+    }
+  }
+  static __device__ __forceinline__ Cell update(const Cell &c, const Q4T<T> &fxl,
+                                                const Q4T<T> &fxr, const Q4T<T> &fyb,
+                                                const Q4T<T> &fyt) {
+    Cell o;
+    for (int v = 0; v < 4; v++) {
+      float e0, e1, e2, e3;
+      float s = two_sum(c.hi[v], fxl.v[v], e0);
+      s = two_sum(s, -fxr.v[v], e1);
+      s = two_sum(s, fyb.v[v], e2);
+      s = two_sum(s, -fyt.v[v], e3);
+      const float lo = c.lo[v] + (((e0 + e1) + e2) + e3);
+      // renormalise (Fast2Sum, |s| >= |lo|)
+      const float hi = s + lo;
+      o.hi[v] = hi;
+      o.lo[v] = lo - (hi - s);
+    }
+    return o;
+  }
+};
+
 __device__ __forceinline__ double &at(const Grid &g, int v, int i, int j) {
-  return g.p[v * g.plane + (long long)j * g.pitch + i];
+  return static_cast<double *>(g.p)[v * g.plane + (long long)j * g.pitch + i];
 }
 
 // Ghost-index map of MakeBoundariesFunctor (HydroRunFunctors.h:1876-2024) in
@@ -56,7 +184,15 @@ __device__ __forceinline__ double wave_max(double x) {
   return x;
 }
 
+__device__ __forceinline__ float wave_max(float x) {
+  for (int off = 32; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off));
+  return x;
+}
+
 // ------------------------------------------------------------------ init
+// Init*Functor bodies in FP64 (HydroRunFunctors.h:1347-1827); the split path
+// stores the FP64 value as hi + lo.
+template <class P>
 __global__ void init_kernel(e2d_params p, Grid U, unsigned int *count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -131,13 +267,11 @@ __global__ void init_kernel(e2d_params p, Grid U, unsigned int *count) {
       e = 0.14 / (gamma0 - 1.0);
     }
   }
-  at(U, ID, i, j) = d;
-  at(U, IP, i, j) = e;
-  at(U, IU, i, j) = mx;
-  at(U, IV, i, j) = my;
+  P::store(U, (long long)j * U.pitch + i, P::from_double(d, e, mx, my));
 }
 
 // InitBlastFunctor, TagCorrectTotalEnergy (:1507-1523)
+template <class P>
 __global__ void blast_energy_kernel(e2d_params p, Grid U, double e_inside) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -147,12 +281,17 @@ __global__ void blast_energy_kernel(e2d_params p, Grid U, double e_inside) {
   const double radius2 = p.blast_radius * p.blast_radius;
   const double d2 = (x - p.blast_center_x) * (x - p.blast_center_x) +
                     (y - p.blast_center_y) * (y - p.blast_center_y);
-  if (d2 < radius2) at(U, IP, i, j) = e_inside;
+  if (d2 < radius2) {
+    const long long off = (long long)j * U.pitch + i;
+    typename P::Cell c = P::load(U, off);
+    P::store(U, off, P::from_double(P::value(c, 0), e_inside, P::value(c, 2), P::value(c, 3)));
+  }
 }
 
 // --------------------------------------------------------------- ghosts
 // One thread per ghost cell: 2 rows at the bottom and top (all columns), then
 // 2 columns on each side of every interior row.
+template <class P>
 __global__ void boundaries_kernel(KParams kp, Grid U) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = 4 * kp.isize;
@@ -171,31 +310,30 @@ __global__ void boundaries_kernel(KParams kp, Grid U) {
   bool fx, fy;
   const int i0 = ghost_map(i, kp.nx, kp.bcxmin, kp.bcxmax, fx);
   const int j0 = ghost_map(j, kp.ny, kp.bcymin, kp.bcymax, fy);
-  const double d = at(U, ID, i0, j0);
-  const double e = at(U, IP, i0, j0);
-  double mx = at(U, IU, i0, j0);
-  double my = at(U, IV, i0, j0);
-  if (fx) mx = -mx;
-  if (fy) my = -my;
-  at(U, ID, i, j) = d;
-  at(U, IP, i, j) = e;
-  at(U, IU, i, j) = mx;
-  at(U, IV, i, j) = my;
+  typename P::Cell c = P::load(U, (long long)j0 * U.pitch + i0);
+  if (fx) P::negate(c, IU);
+  if (fy) P::negate(c, IV);
+  P::store(U, (long long)j * U.pitch + i, c);
 }
 
 // ------------------------------------------------------------------- dt
+template <class P>
 __global__ void dt_reduce_kernel(KParams kp, Grid U, unsigned long long *invdt_bits) {
+  using T = typename P::T;
   const long long n = (long long)kp.nx * kp.ny;
-  double m = 0.0;
+  const T dx = (T)kp.dx, dy = (T)kp.dy;
+  T m = T(0.0);
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
        idx += (long long)gridDim.x * blockDim.x) {
     const int j = 2 + (int)(idx / kp.nx);
     const int i = 2 + (int)(idx - (long long)(j - 2) * kp.nx);
-    m = fmax(m, cfl_rate(kp.s, at(U, ID, i, j), at(U, IP, i, j), at(U, IU, i, j),
-                         at(U, IV, i, j), kp.dx, kp.dy));
+    Q4T<T> q;
+    P::prim(kp, P::load(U, (long long)j * U.pitch + i), q);
+    m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, dx, dy));
   }
   m = wave_max(m);
-  if ((threadIdx.x & 63) == 0 && m == m) atomicMax(invdt_bits, dbits(m));
+  const double md = (double)m;
+  if ((threadIdx.x & 63) == 0 && md == md && md > 0.0) atomicMax(invdt_bits, dbits(md));
 }
 
 // Starts a run loop: dt of the pending step from the reduced invDt, with the
@@ -301,32 +439,37 @@ __global__ void update_kernel(KParams kp, Grid U, Grid Fx, Grid Fy) {
 // block, folds the next step's CFL rate of every updated cell into a wave max,
 // and the last block to finish advances t, nStep and the next dt.
 
-__device__ __forceinline__ Q4 shfl_up1(const Q4 &a) {
-  Q4 r;
+template <class T>
+__device__ __forceinline__ Q4T<T> shfl_up1(const Q4T<T> &a) {
+  Q4T<T> r;
   for (int v = 0; v < 4; v++) r.v[v] = __shfl_up(a.v[v], 1);
   return r;
 }
 
-__device__ __forceinline__ Q4 shfl_dn1(const Q4 &a) {
-  Q4 r;
+template <class T>
+__device__ __forceinline__ Q4T<T> shfl_dn1(const Q4T<T> &a) {
+  Q4T<T> r;
   for (int v = 0; v < 4; v++) r.v[v] = __shfl_down(a.v[v], 1);
   return r;
 }
 
-template <int RS>
+template <int RS, class P>
 __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in, Grid out,
                                                             double dt_arg, StepCtl *ctl,
                                                             int seg_rows, int nstrips, int nwaves) {
+  using T = typename P::T;
+  using Cell = typename P::Cell;
   if (ctl && !ctl->active) return;  // loop finished: nothing to do (uniform)
-  const Scheme &s = kp.s;
+  const SchemeT<T> &s = P::scheme(kp);
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6);
-  double wmax = 0.0;
+  T wmax = T(0.0);
 
   if (wave < nwaves) {
     const double dt = ctl ? ctl->dt : dt_arg;
-    const double dtdx = dt / kp.dx;
-    const double dtdy = dt / kp.dy;
+    const T dtdx = (T)(dt / kp.dx);
+    const T dtdy = (T)(dt / kp.dy);
+    const T dxr = (T)kp.dx, dyr = (T)kp.dy;
 
     const int strip = wave % nstrips;
     const int seg = wave / nstrips;
@@ -344,28 +487,24 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
                            (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
     const bool out_col = lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1;
 
-    auto load_row = [&](int r, double u[4]) {
+    auto load_row = [&](int r) -> Cell {
       bool flipy;
       const int rc = min(r, kp.ny + 3);
       const int r0 = ghost_map(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
-      const long long off = (long long)r0 * in.pitch + c0;
-      u[0] = in.p[off];
-      u[1] = in.p[in.plane + off];
-      u[2] = in.p[2 * in.plane + off];
-      u[3] = in.p[3 * in.plane + off];
-      if (flipx) u[2] = -u[2];
-      if (flipy) u[3] = -u[3];
+      Cell c = P::load(in, (long long)r0 * in.pitch + c0);
+      if (flipx) P::negate(c, IU);
+      if (flipy) P::negate(c, IV);
+      return c;
     };
 
     const int nrows = seg_rows + 4;
-    double un[4], ua[4] = {0, 0, 0, 0}, ub[4] = {0, 0, 0, 0};
-    Q4 qm2 = {}, qm1 = {}, fymax_c = {}, fx_c = {}, fy_c = {};
-    load_row(j_base - 2, un);
+    Cell un = load_row(j_base - 2), ua = un, ub = un;
+    Q4T<T> qm2 = {}, qm1 = {}, fymax_c = {}, fx_c = {}, fy_c = {};
 
     for (int k = 0; k < nrows; ++k) {
       const int r = j_base - 2 + k;
-      double uc[4] = {un[0], un[1], un[2], un[3]};
-      if (k + 1 < nrows) load_row(r + 1, un);
+      const Cell uc = un;
+      if (k + 1 < nrows) un = load_row(r + 1);
 
       // ghost cells of this row owned by this wave: materialise them in both arrays
       const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
@@ -374,61 +513,54 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
                              (seg == nsegs - 1 && r > kp.ny + 1 && r <= kp.ny + 3);
       if ((col_ghost || row_ghost) && col_owned && row_owned) {
         const long long off = (long long)r * in.pitch + col;
-        for (int v = 0; v < 4; v++) {
-          in.p[v * in.plane + off] = uc[v];
-          out.p[v * out.plane + off] = uc[v];
-        }
+        P::store(in, off, uc);
+        P::store(out, off, uc);
       }
 
-      Q4 q0;
-      prim_no_c(s, uc[0], uc[1], uc[2], uc[3], q0);
+      Q4T<T> q0;
+      P::prim(kp, uc, q0);
 
       if (k >= 2) {
         // row a = r - 1: slopes, source terms, y-face states
-        const Q4 qpx = shfl_dn1(qm1);
-        const Q4 qmx = shfl_up1(qm1);
-        Q4 dqX, dqY;
+        const Q4T<T> qpx = shfl_dn1(qm1);
+        const Q4T<T> qmx = shfl_up1(qm1);
+        Q4T<T> dqX, dqY;
         for (int v = 0; v < 4; v++) {
           dqX.v[v] = slope1(s, qm1.v[v], qpx.v[v], qmx.v[v]);
           dqY.v[v] = slope1(s, qm1.v[v], q0.v[v], qm2.v[v]);
         }
-        const Sources src = trace_sources(s, qm1, dqX, dqY);
-        Q4 fymin, fymax;
+        const SourcesT<T> src = trace_sources(s, qm1, dqX, dqY);
+        Q4T<T> fymin, fymax;
         trace_face(s, qm1, dqY, src, dtdy, false, fymin);
         trace_face(s, qm1, dqY, src, dtdy, true, fymax);
         if (k >= 3) {
           // y face between rows a-1 and a
-          double f[4];
+          T f[4];
           riemann<RS>(s, fymax_c, fymin, IV, IU, f);
-          Q4 fya;
+          Q4T<T> fya;
           for (int v = 0; v < 4; v++) fya.v[v] = f[v] * dtdy;
-          Q4 fxa = {};
+          Q4T<T> fxa = {};
           if (k <= seg_rows + 2) {
             // x faces of row a: left state from lane-1's XMAX face
-            Q4 fxmin, fxmax;
+            Q4T<T> fxmin, fxmax;
             trace_face(s, qm1, dqX, src, dtdx, false, fxmin);
             trace_face(s, qm1, dqX, src, dtdx, true, fxmax);
-            const Q4 ql = shfl_up1(fxmax);
+            const Q4T<T> ql = shfl_up1(fxmax);
             riemann<RS>(s, ql, fxmin, IU, IV, f);
             for (int v = 0; v < 4; v++) fxa.v[v] = f[v] * dtdx;
           }
           if (k >= 4) {
             // update row b = a - 1
             const int b = r - 2;
-            const Q4 fxr = shfl_dn1(fx_c);
-            double un4[4];
-            for (int v = 0; v < 4; v++) {
-              double u = ub[v];
-              u += fx_c.v[v];
-              u -= fxr.v[v];
-              u += fy_c.v[v];
-              u -= fya.v[v];
-              un4[v] = u;
-            }
+            const Q4T<T> fxr = shfl_dn1(fx_c);
+            const Cell unew = P::update(ub, fx_c, fxr, fy_c, fya);
             if (out_col && b <= kp.ny + 1) {
-              const long long off = (long long)b * out.pitch + col;
-              for (int v = 0; v < 4; v++) out.p[v * out.plane + off] = un4[v];
-              if (ctl) wmax = fmax(wmax, cfl_rate(s, un4[0], un4[1], un4[2], un4[3], kp.dx, kp.dy));
+              P::store(out, (long long)b * out.pitch + col, unew);
+              if (ctl) {
+                Q4T<T> qn;
+                P::prim(kp, unew, qn);
+                wmax = Ops<T>::mx(wmax, cfl_rate_q(s, qn, dxr, dyr));
+              }
             }
           }
           fx_c = fxa;
@@ -438,18 +570,16 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
       }
       qm2 = qm1;
       qm1 = q0;
-      for (int v = 0; v < 4; v++) {
-        ub[v] = ua[v];
-        ua[v] = uc[v];
-      }
+      ub = ua;
+      ua = uc;
     }
   }
 
   if (!ctl) return;
   // run-loop epilogue: next-step invDt (exact max, order independent) and the
   // last block advances the loop state
-  wmax = wave_max(wmax);
-  if (lane == 0 && wmax == wmax && wmax > 0.0) atomicMax(&ctl->invdt, dbits(wmax));
+  const double wm = (double)wave_max(wmax);
+  if (lane == 0 && wm == wm && wm > 0.0) atomicMax(&ctl->invdt, dbits(wm));
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -473,15 +603,18 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
 }
 
 // -------------------------------------------------- conserved-variable sums
-// Deterministic two-level sum: block b sums rows b, b+G, ... in a fixed tree.
+// Deterministic two-level FP64 sum: block b sums rows b, b+G, ... in a fixed tree.
 constexpr int kSumBlocks = 512;
 
+template <class P>
 __global__ void sums_partial_kernel(KParams kp, Grid U, double *partials) {
   __shared__ double red[4][256];
   double acc[4] = {0, 0, 0, 0};
   for (int j = 2 + blockIdx.x; j <= kp.ny + 1; j += gridDim.x)
-    for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x)
-      for (int v = 0; v < 4; v++) acc[v] += at(U, v, i, j);
+    for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x) {
+      const typename P::Cell c = P::load(U, (long long)j * U.pitch + i);
+      for (int v = 0; v < 4; v++) acc[v] += P::value(c, v);
+    }
   for (int v = 0; v < 4; v++) red[v][threadIdx.x] = acc[v];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -520,9 +653,35 @@ dim3 grid2d(int nx, int ny, dim3 blk) {
   return dim3((unsigned)((nx + blk.x - 1) / blk.x), (unsigned)((ny + blk.y - 1) / blk.y));
 }
 
+template <int RS, class P>
+void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl, int seg_rows,
+                    hipStream_t st) {
+  const int nstrips = (kp.nx + kOutCols - 1) / kOutCols;
+  const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
+  const int nwaves = nstrips * nsegs;
+  const int waves_per_block = kBlock / kLanes;
+  const int blocks = (nwaves + waves_per_block - 1) / waves_per_block;
+  hipLaunchKernelGGL((step_fused_kernel<RS, P>), dim3(blocks), dim3(kBlock), 0, st, kp, in, out,
+                     dt, ctl, seg_rows, nstrips, nwaves);
+}
+
+template <class P>
+void launch_fused_p(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl, int seg_rows,
+                    hipStream_t st) {
+  if (kp.s.riemann == 2)
+    launch_fused_t<2, P>(kp, in, out, dt, ctl, seg_rows, st);
+  else if (kp.s.riemann == 1)
+    launch_fused_t<1, P>(kp, in, out, dt, ctl, seg_rows, st);
+  else
+    launch_fused_t<0, P>(kp, in, out, dt, ctl, seg_rows, st);
+}
+
 }  // namespace
 
-KParams make_kparams(const e2d_params &p) {
+int grid_planes(int precision) { return precision == E2D_PRECISION_FP32_SPLIT ? 8 : 4; }
+int grid_scalar_bytes(int precision) { return precision == E2D_PRECISION_FP32_SPLIT ? 4 : 8; }
+
+KParams make_kparams(const e2d_params &p, int precision) {
   KParams k{};
   k.s.gamma0 = p.gamma0;
   k.s.gamma6 = p.gamma6;
@@ -534,6 +693,17 @@ KParams make_kparams(const e2d_params &p) {
   k.s.entho = 1.0 / (p.gamma0 - 1.0);
   k.s.smallc2 = p.smallc * p.smallc;
   k.s.riemann = p.honor_riemann ? p.riemannSolverType : 2;
+  k.sf.gamma0 = (float)k.s.gamma0;
+  k.sf.gamma6 = (float)k.s.gamma6;
+  k.sf.smallr = (float)k.s.smallr;
+  k.sf.smallc = (float)k.s.smallc;
+  k.sf.smallp = (float)k.s.smallp;
+  k.sf.smallpp = (float)k.s.smallpp;
+  k.sf.slope_type = (float)k.s.slope_type;
+  k.sf.entho = (float)k.s.entho;
+  k.sf.smallc2 = (float)k.s.smallc2;
+  k.sf.riemann = k.s.riemann;
+  k.prec = precision;
   k.nx = p.nx;
   k.ny = p.ny;
   k.isize = p.isize;
@@ -545,27 +715,40 @@ KParams make_kparams(const e2d_params &p) {
   k.nStepmax = p.nStepmax;
   k.dx = p.dx;
   k.dy = p.dy;
+  k.dxf = (float)p.dx;
+  k.dyf = (float)p.dy;
   k.cfl = p.cfl;
   k.tEnd = p.tEnd;
+  k.kin_frac = 0.1f;
   return k;
 }
 
-hipError_t launch_init(const e2d_params &p, Grid U, unsigned int *count_dev, hipStream_t st) {
+#define E2D_DISPATCH_P(kp, KERNEL, ...)                   \
+  do {                                                    \
+    if ((kp).prec == E2D_PRECISION_FP32_SPLIT)            \
+      hipLaunchKernelGGL(KERNEL<PSplit>, __VA_ARGS__);    \
+    else                                                  \
+      hipLaunchKernelGGL(KERNEL<P64>, __VA_ARGS__);       \
+  } while (0)
+
+hipError_t launch_init(const KParams &kp, const e2d_params &p, Grid U, unsigned int *count_dev,
+                       hipStream_t st) {
   const dim3 blk(64, 4);
-  hipLaunchKernelGGL(init_kernel, grid2d(p.isize, p.jsize, blk), blk, 0, st, p, U, count_dev);
+  E2D_DISPATCH_P(kp, init_kernel, grid2d(p.isize, p.jsize, blk), blk, 0, st, p, U, count_dev);
   return hipGetLastError();
 }
 
-hipError_t launch_blast_energy(const e2d_params &p, Grid U, double e_inside, hipStream_t st) {
+hipError_t launch_blast_energy(const KParams &kp, const e2d_params &p, Grid U, double e_inside,
+                               hipStream_t st) {
   const dim3 blk(64, 4);
-  hipLaunchKernelGGL(blast_energy_kernel, grid2d(p.isize, p.jsize, blk), blk, 0, st, p, U,
-                     e_inside);
+  E2D_DISPATCH_P(kp, blast_energy_kernel, grid2d(p.isize, p.jsize, blk), blk, 0, st, p, U,
+                 e_inside);
   return hipGetLastError();
 }
 
 hipError_t launch_boundaries(const KParams &kp, Grid U, hipStream_t st) {
   const int n = 4 * kp.isize + 4 * kp.ny;
-  hipLaunchKernelGGL(boundaries_kernel, dim3((n + 255) / 256), dim3(256), 0, st, kp, U);
+  E2D_DISPATCH_P(kp, boundaries_kernel, dim3((n + 255) / 256), dim3(256), 0, st, kp, U);
   return hipGetLastError();
 }
 
@@ -574,8 +757,8 @@ hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt
   const long long n = (long long)kp.nx * kp.ny;
   long long blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(dt_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U,
-                     invdt_bits);
+  E2D_DISPATCH_P(kp, dt_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U,
+                 invdt_bits);
   return hipGetLastError();
 }
 
@@ -612,38 +795,29 @@ hipError_t launch_update(const KParams &kp, Grid U, Grid Fx, Grid Fy, hipStream_
 hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl,
                              int seg_rows, hipStream_t st) {
   if (seg_rows < 1) seg_rows = 1;
-  const int nstrips = (kp.nx + kOutCols - 1) / kOutCols;
-  const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
-  const int nwaves = nstrips * nsegs;
-  const int waves_per_block = kBlock / kLanes;
-  const int blocks = (nwaves + waves_per_block - 1) / waves_per_block;
-  if (kp.s.riemann == 2)
-    hipLaunchKernelGGL(step_fused_kernel<2>, dim3(blocks), dim3(kBlock), 0, st, kp, in, out, dt,
-                       ctl, seg_rows, nstrips, nwaves);
-  else if (kp.s.riemann == 1)
-    hipLaunchKernelGGL(step_fused_kernel<1>, dim3(blocks), dim3(kBlock), 0, st, kp, in, out, dt,
-                       ctl, seg_rows, nstrips, nwaves);
+  if (kp.prec == E2D_PRECISION_FP32_SPLIT)
+    launch_fused_p<PSplit>(kp, in, out, dt, ctl, seg_rows, st);
   else
-    hipLaunchKernelGGL(step_fused_kernel<0>, dim3(blocks), dim3(kBlock), 0, st, kp, in, out, dt,
-                       ctl, seg_rows, nstrips, nwaves);
+    launch_fused_p<P64>(kp, in, out, dt, ctl, seg_rows, st);
   return hipGetLastError();
 }
 
 int sums_partials_needed(const KParams &) { return kSumBlocks * 4; }
 
-hipError_t launch_copy(Grid src, Grid dst, hipStream_t st) {
-  const long long n2 = src.plane * 4 / 2;  // planes are 64-element aligned
-  long long blocks = (n2 + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
-                     reinterpret_cast<const double2 *>(src.p), reinterpret_cast<double2 *>(dst.p),
-                     n2);
+hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st) {
+  E2D_DISPATCH_P(kp, sums_partial_kernel, dim3(kSumBlocks), dim3(256), 0, st, kp, U, partials);
+  hipLaunchKernelGGL(sums_final_kernel, dim3(1), dim3(256), 0, st, partials, kSumBlocks, out4);
   return hipGetLastError();
 }
 
-hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st) {
-  hipLaunchKernelGGL(sums_partial_kernel, dim3(kSumBlocks), dim3(256), 0, st, kp, U, partials);
-  hipLaunchKernelGGL(sums_final_kernel, dim3(1), dim3(256), 0, st, partials, kSumBlocks, out4);
+hipError_t launch_copy(const KParams &kp, Grid src, Grid dst, hipStream_t st) {
+  // every plane starts on a 512-byte boundary, so the array is whole 16-byte units
+  const long long bytes = src.plane * grid_planes(kp.prec) * grid_scalar_bytes(kp.prec);
+  const long long n2 = bytes / 16;
+  long long blocks = (n2 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
+                     static_cast<const double2 *>(src.p), static_cast<double2 *>(dst.p), n2);
   return hipGetLastError();
 }
 

@@ -1,4 +1,4 @@
-// kernels.hpp -- host-side declarations of the FP64 HIP kernels' launchers.
+// kernels.hpp -- host-side declarations of the HIP kernels' launchers.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,21 +10,30 @@
 
 namespace e2d {
 
-// One state array on the device: four SoA planes [var][j][i], row pitch in
-// elements, plane stride in elements.  Element (v,i,j) at p[v*plane + j*pitch + i].
+// One state array on the device, as SoA planes [plane][j][i] with a row pitch
+// and a plane stride in elements of the storage scalar:
+//   FP64        4 planes of double, plane v = variable v
+//   FP32 split  8 planes of float, plane 2v = hi part, 2v+1 = lo part of
+//               variable v (value = hi + lo, |lo| <= ulp(hi)/2)
 struct Grid {
-  double *p;
+  void *p;
   int pitch;
   long long plane;
 };
 
 // Everything a kernel needs from HydroParams, by value.
 struct KParams {
-  Scheme s;
+  Scheme s;             // FP64 constants
+  SchemeT<float> sf;    // the same constants rounded to float (split path)
+  int prec;             // E2D_PRECISION_*
   int nx, ny, isize, jsize;
   int bcxmin, bcxmax, bcymin, bcymax;
   int nStepmax;
   double dx, dy, cfl, tEnd;
+  float dxf, dyf;
+  // split path: primitive conversion redone in FP64 where the internal energy
+  // is below this fraction of the total energy (p < kin_frac * (gamma-1) * E)
+  float kin_frac;
 };
 
 // Run-loop control block kept in device memory (time loop of main.cpp:100-145
@@ -45,14 +54,20 @@ constexpr int kLanes = 64;
 constexpr int kOutCols = 60;
 constexpr int kBlock = 256;
 
-KParams make_kparams(const e2d_params &p);
+KParams make_kparams(const e2d_params &p, int precision);
+// planes of one state array and bytes of its storage scalar
+int grid_planes(int precision);
+int grid_scalar_bytes(int precision);
 
-hipError_t launch_init(const e2d_params &p, Grid U, unsigned int *count_dev, hipStream_t st);
-hipError_t launch_blast_energy(const e2d_params &p, Grid U, double e_inside, hipStream_t st);
+hipError_t launch_init(const KParams &kp, const e2d_params &p, Grid U, unsigned int *count_dev,
+                       hipStream_t st);
+hipError_t launch_blast_energy(const KParams &kp, const e2d_params &p, Grid U, double e_inside,
+                               hipStream_t st);
 hipError_t launch_boundaries(const KParams &kp, Grid U, hipStream_t st);
 hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt_bits,
                             hipStream_t st);
 hipError_t launch_dt_finalize(const KParams &kp, StepCtl *ctl, hipStream_t st);
+// reference pass schedule (FP64 only)
 hipError_t launch_prim(const KParams &kp, Grid U, Grid Q, hipStream_t st);
 hipError_t launch_fluxes(const KParams &kp, Grid Q, Grid Fx, Grid Fy, double dtdx, double dtdy,
                          hipStream_t st);
@@ -62,7 +77,7 @@ hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, St
                              int seg_rows, hipStream_t st);
 hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st);
 int sums_partials_needed(const KParams &kp);
-// whole-array copy (all four planes, ghosts and padding included)
-hipError_t launch_copy(Grid src, Grid dst, hipStream_t st);
+// whole-array copy (all planes, ghosts and padding included)
+hipError_t launch_copy(const KParams &kp, Grid src, Grid dst, hipStream_t st);
 
 }  // namespace e2d

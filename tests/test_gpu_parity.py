@@ -45,6 +45,13 @@ CASES = {
                             boundary_type_xmin=1, boundary_type_xmax=2, boundary_type_ymin=3,
                             boundary_type_ymax=3, slope_type=0.0, blast_radius=0.25),
     "tend_clamp": dict(nx=40, ny=40, problemType=0, cfl=0.8, nStepmax=1000, tEnd=0.05),
+    "four_quadrant_hll": dict(nx=64, ny=48, problemType=2, cfl=0.8, nStepmax=40, tEnd=1e9,
+                              boundary_type_xmin=2, boundary_type_xmax=2, boundary_type_ymin=2,
+                              boundary_type_ymax=2, honor_riemann=1, riemannSolverType=1),
+    "blast_approx": dict(nx=48, ny=56, problemType=1, cfl=0.6, nStepmax=40, tEnd=1e9,
+                         honor_riemann=1, riemannSolverType=0, blast_radius=0.15),
+    "implode_approx_reflect": dict(nx=40, ny=40, problemType=0, cfl=0.7, nStepmax=40,
+                                   tEnd=1e9, honor_riemann=1, riemannSolverType=0),
     "sedov_energy": dict(nx=50, ny=50, problemType=1, cfl=0.7, nStepmax=30, tEnd=1e9,
                          blast_total_energy_inside=1.0, blast_radius=0.05,
                          blast_density_out=1.0),

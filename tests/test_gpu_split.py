@@ -1,0 +1,118 @@
+"""FP32 split-precision path against the FP64 path on the GPU.
+
+Gates (north star): relative L1 density error <= 1e-4 against FP64 after N
+steps, and total mass and energy conserved to ~1e-12 (FP64 sums of hi + lo).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from euler2d_kokkos_amd import (E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64, E2D_U, E2D_U2,
+                                E2DError, HydroParams, HydroRun)
+from euler2d_kokkos_amd import E2D_SCHEDULE_PASSES
+
+pytestmark = pytest.mark.gpu
+
+L1_TOL = 1e-4
+
+CASES = {
+    "implode128": dict(nx=128, ny=128, problemType=0, cfl=0.8, nStepmax=300, tEnd=1e9),
+    "blast96x144": dict(nx=96, ny=144, problemType=1, cfl=0.7, nStepmax=300, tEnd=1e9, ymax=1.5,
+                        blast_radius=0.1, blast_center_x=0.5, blast_center_y=0.75,
+                        blast_density_in=1.0, blast_density_out=1.0),
+    "four_quadrant_hll": dict(nx=128, ny=128, problemType=2, cfl=0.8, nStepmax=200, tEnd=1e9,
+                              boundary_type_xmin=2, boundary_type_xmax=2, boundary_type_ymin=2,
+                              boundary_type_ymax=2, honor_riemann=1, riemannSolverType=1),
+    "shocked_bubble": dict(nx=192, ny=96, problemType=4, cfl=0.8, nStepmax=200, tEnd=1e9,
+                           xmax=2.0, boundary_type_xmin=2, boundary_type_xmax=2,
+                           sb_preshock_density=1.0, sb_preshock_pressure=1.0,
+                           sb_postshock_density=3.857142857142857,
+                           sb_postshock_pressure=10.333333333333332,
+                           sb_postshock_velocity=2.6293687924887177, sb_shock_loc=0.1,
+                           sb_bubble_density=0.1, sb_bubble_pressure=1.0, sb_bubble_radius=0.15,
+                           sb_bubble_center_x=0.4, sb_bubble_center_y=0.5),
+}
+
+
+def _runs(case):
+    p = O.default_params(**CASES[case])
+    r64 = HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP64)
+    r32 = HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP32_SPLIT)
+    return p, r64, r32
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_split_close_to_fp64(gpu_available, case):
+    p, r64, r32 = _runs(case)
+    s0 = r32.conserved_sums(E2D_U)
+    r64.run()
+    r32.run()
+    assert r32.nStep == r64.nStep == p.nStepmax
+    a = r32.get_state(r32.current())
+    b = r64.get_state(r64.current())
+    err = O.rel_l1(O.interior(a), O.interior(b))
+    drift = (r32.conserved_sums(r32.current()) - s0) / np.abs(s0).clip(1e-300)
+    print(f"{case}: steps={r32.nStep} t32={r32.t!r} t64={r64.t!r} relL1={err.tolist()} "
+          f"drift={drift.tolist()}")
+    assert err[0] <= L1_TOL, err
+    assert abs(r32.t - r64.t) <= 1e-4 * r64.t
+
+
+@pytest.mark.parametrize("bc,conserved", [(1, (0, 1)), (3, (0, 1, 2, 3))])
+def test_split_conservation(gpu_available, bc, conserved):
+    p = O.default_params(nx=160, ny=120, problemType=1, cfl=0.7, nStepmax=1000, tEnd=1e9,
+                         blast_radius=0.12, blast_center_x=0.45, blast_center_y=0.55,
+                         boundary_type_xmin=bc, boundary_type_xmax=bc, boundary_type_ymin=bc,
+                         boundary_type_ymax=bc)
+    r = HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP32_SPLIT)
+    s0 = r.conserved_sums(E2D_U)
+    r.run()
+    s1 = r.conserved_sums(r.current())
+    # drift relative to the L1 norm of each variable (momentum starts at zero)
+    scale = np.abs(r.get_state(r.current())[2:-2, 2:-2, :]).reshape(-1, 4).sum(0)
+    rel = np.abs(s1 - s0) / scale
+    print(f"bc={bc} steps={r.nStep} drift={rel.tolist()}")
+    for v in conserved:
+        assert rel[v] <= 1e-12, (v, rel)
+
+
+def test_split_uniform_state(gpu_available):
+    p = O.default_params(nx=70, ny=50, cfl=0.8, nStepmax=50, tEnd=1e9, boundary_type_xmin=3,
+                         boundary_type_xmax=3, boundary_type_ymin=3, boundary_type_ymax=3)
+    r = HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP32_SPLIT, initialise=False)
+    w = np.array([1.3, 0.7 / 0.4 + 0.5 * 1.3 * (0.4 ** 2 + 0.25 ** 2), 1.3 * 0.4, -1.3 * 0.25])
+    u = np.broadcast_to(w, (p.isize, p.jsize, 4)).copy()
+    r.set_state(E2D_U, u)
+    r.set_state(E2D_U2, u)
+    r.run()
+    got = r.get_state(r.current())[2:-2, 2:-2, :]
+    np.testing.assert_allclose(got, np.broadcast_to(w, got.shape), rtol=1e-6)
+
+
+def test_split_state_roundtrip(gpu_available):
+    p = O.default_params(nx=33, ny=17)
+    r = HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP32_SPLIT)
+    rng = np.random.default_rng(3)
+    a = rng.random((p.isize, p.jsize, 4)) + 0.5
+    r.set_state(E2D_U, a)
+    b = r.get_state(E2D_U)
+    # hi + lo holds 48 significant bits
+    np.testing.assert_allclose(b, a, rtol=2.0 ** -46, atol=0)
+
+
+def test_split_rejects_pass_schedule(gpu_available):
+    p = O.default_params(nx=16, ny=16, nStepmax=2, tEnd=1e9)
+    r = HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP32_SPLIT,
+                 schedule=E2D_SCHEDULE_PASSES)
+    with pytest.raises(E2DError):
+        r.godunov_unsplit(0, 1e-3)
+
+
+def test_split_tend_clamp(gpu_available):
+    p = O.default_params(nx=40, ny=40, problemType=0, cfl=0.8, nStepmax=100000, tEnd=0.05)
+    r64 = HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP64)
+    r32 = HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP32_SPLIT)
+    r64.run()
+    r32.run()
+    assert r64.t == pytest.approx(0.05, rel=1e-15) and r32.t == pytest.approx(0.05, rel=1e-15)
+    assert abs(r32.nStep - r64.nStep) <= 1
