@@ -41,6 +41,10 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the FP32 split-precision comparison run")
+    ap.add_argument("--tile", type=int, default=14,
+                    help="rows marched by one wavefront of the step kernel")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="K",
                     help="only launch the plain whole-array copy kernel K times (the known-size "
                          "reference for rocprof FETCH_SIZE / WRITE_SIZE) and exit")
@@ -72,6 +76,40 @@ def cpu_baseline(deck: Path, seconds: float) -> dict:
                       f"{el:.1f} s"}
 
 
+def split_comparison(params, run64, steps, tile, cells, fp64_ms) -> dict:
+    """Same workload in FP32 split precision: throughput, L1 error against the
+    FP64 run's final state, and FP64-summed drift of mass and energy."""
+    import numpy as np
+
+    from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_U, HydroRun
+
+    run = HydroRun(params, precision=E2D_PRECISION_FP32_SPLIT)
+    run.set_tile(tile)
+    run.run(20)
+    run.synchronize()
+    run.init_problem()
+    s0 = run.conserved_sums(E2D_U)
+    t0 = time.perf_counter()
+    ms, executed = run.time_loop(steps)
+    run.synchronize()
+    wall = time.perf_counter() - t0
+    a = run.get_state(run.current())[2:-2, 2:-2, :].reshape(-1, 4)
+    b = run64.get_state(run64.current())[2:-2, 2:-2, :].reshape(-1, 4)
+    l1 = np.abs(a - b).sum(0) / np.abs(b).sum(0)
+    s1 = run.conserved_sums(run.current())
+    value = cells * executed / wall / 1e6
+    per_step = ms / executed
+    return {"value": round(value, 2), "unit": "Mcell-updates/s",
+            "speedup_vs_fp64": round(fp64_ms / per_step, 3),
+            "kernel_ms_per_step": round(per_step, 5),
+            "roofline_frac": round(BYTES_PER_CELL * cells / (per_step * 1e-3) / 1e9
+                                   / HBM_PEAK_GBS, 4),
+            "rel_l1_vs_fp64": [float(x) for x in l1],
+            "mass_drift": float(abs(s1[0] - s0[0]) / abs(s0[0])),
+            "energy_drift": float(abs(s1[1] - s0[1]) / abs(s0[1])),
+            "final_t": run.t}
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -80,12 +118,14 @@ def main() -> int:
                           "n_gpus": args.gpus}))
         return 2
 
-    from euler2d_kokkos_amd import E2D_SCHEDULE_FUSED, HydroParams, HydroRun, device_info
+    from euler2d_kokkos_amd import (E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64,
+                                    E2D_SCHEDULE_FUSED, HydroParams, HydroRun, device_info)
 
     dev, ncu = device_info()
     params = HydroParams.from_ini(DECK)
     params.nStepmax = max(params.nStepmax, args.steps)
     run = HydroRun(params, schedule=E2D_SCHEDULE_FUSED)
+    run.set_tile(args.tile)
 
     if args.calibrate_copy > 0:
         from euler2d_kokkos_amd import E2D_U, E2D_U2
@@ -138,6 +178,9 @@ def main() -> int:
         "kernel_ms_per_step": round(per_step_ms, 5),
         "final_t": run.t,
     }
+    if not args.no_split:
+        line["fp32_split"] = split_comparison(params, run, args.steps, args.tile, cells,
+                                              per_step_ms)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(DECK, args.cpu_seconds)
     else:
