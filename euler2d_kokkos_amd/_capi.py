@@ -116,6 +116,20 @@ _PROTOTYPES = {
     "e2d_synchronize": (C.c_int, [C.c_void_p]),
     "e2d_time_loop": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
                                 C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "e2d_batch_create": (C.c_int, [C.POINTER(E2DParams), C.c_int, C.c_int,
+                                   C.POINTER(C.c_void_p)]),
+    "e2d_batch_destroy": (C.c_int, [C.c_void_p]),
+    "e2d_batch_init": (C.c_int, [C.c_void_p]),
+    "e2d_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
+    "e2d_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                C.POINTER(C.c_int)]),
+    "e2d_batch_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int)]),
+    "e2d_batch_copy_state": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "e2d_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int)]),
+    "e2d_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_double)]),
 }
 
 EXPORTED = tuple(_PROTOTYPES)

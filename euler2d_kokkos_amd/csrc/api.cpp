@@ -721,4 +721,293 @@ int e2d_synchronize(e2d_run *r) {
   return E2D_OK;
 }
 
+// ------------------------------------------------------------------ batch
+}  // extern "C"
+
+struct e2d_batch {
+  std::vector<e2d_run *> runs;
+  int precision = 0;
+  int seg_rows = 14;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  e2d::BatchInst *insts = nullptr;   // device
+  e2d::StepCtl *ctls = nullptr;      // device
+  unsigned int *blocks_done = nullptr;
+  int nwaves = 0;
+  long long cells = 0;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+  int graph_tile = 0;
+};
+
+namespace {
+
+constexpr int kGraphSteps = 32;
+
+int batch_check(e2d_batch *b) {
+  if (!b) return fail(E2D_ERR_ARG, "null e2d_batch handle");
+  hipError_t e = hipSetDevice(b->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return E2D_OK;
+}
+
+void batch_drop_graph(e2d_batch *b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_steps = 0;
+}
+
+// instance table for the current tile
+int batch_upload_table(e2d_batch *b) {
+  std::vector<e2d::BatchInst> h(b->runs.size());
+  int waves = 0;
+  long long cells = 0;
+  for (size_t i = 0; i < b->runs.size(); ++i) {
+    e2d_run *r = b->runs[i];
+    e2d::BatchInst &bi = h[i];
+    std::memset(&bi, 0, sizeof(bi));
+    bi.kp = r->kp;
+    bi.U = r->U;
+    bi.U2 = r->U2;
+    bi.wave_begin = waves;
+    waves += e2d::batch_waves(r->kp, b->seg_rows, &bi.nstrips);
+    cells += (long long)r->p.nx * r->p.ny;
+  }
+  b->nwaves = waves;
+  b->cells = cells;
+  HIP_TRY(hipMemcpyAsync(b->insts, h.data(), h.size() * sizeof(e2d::BatchInst),
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
+  return E2D_OK;
+}
+
+int batch_launch(e2d_batch *b) {
+  HIP_TRY(e2d::launch_step_batch(b->runs[0]->kp, b->insts, (int)b->runs.size(), b->ctls,
+                                 b->nwaves, b->seg_rows, b->blocks_done, b->stream));
+  return E2D_OK;
+}
+
+int batch_graph(e2d_batch *b) {
+  if (b->graph && b->graph_tile == b->seg_rows) return E2D_OK;
+  batch_drop_graph(b);
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < kGraphSteps; ++k) {
+    int rc = batch_launch(b);
+    if (rc) {
+      (void)hipStreamEndCapture(b->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+  }
+  HIP_TRY(hipStreamEndCapture(b->stream, &g));
+  hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  b->graph_steps = kGraphSteps;
+  b->graph_tile = b->seg_rows;
+  return E2D_OK;
+}
+
+int batch_active(e2d_batch *b, int *active) {
+  std::vector<e2d::StepCtl> c(b->runs.size());
+  HIP_TRY(hipMemcpyAsync(c.data(), b->ctls, c.size() * sizeof(e2d::StepCtl),
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  int n = 0;
+  for (auto &x : c) n += x.active ? 1 : 0;
+  *active = n;
+  return E2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_batch **out) {
+  if (!params || !out || count <= 0) return fail(E2D_ERR_ARG, "bad argument");
+  *out = nullptr;
+  return guarded([&]() -> int {
+    auto *b = new e2d_batch();
+    b->precision = precision;
+    for (int i = 0; i < count; ++i) {
+      e2d_run *r = nullptr;
+      int rc = e2d_create(params + i, precision, &r);
+      if (rc) {
+        const std::string msg = "instance " + std::to_string(i) + ": " + g_last_error;
+        e2d_batch_destroy(b);
+        return fail(rc, msg);
+      }
+      b->runs.push_back(r);
+      if (r->kp.s.riemann != b->runs[0]->kp.s.riemann) {
+        e2d_batch_destroy(b);
+        return fail(E2D_ERR_UNSUPPORTED, "all instances of a batch must use the same Riemann solver");
+      }
+    }
+    b->device = b->runs[0]->device;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      e2d_batch_destroy(b);
+      return hip_fail(e, "hipStreamCreate");
+    }
+    void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
+    if (hipMalloc(&p1, sizeof(e2d::BatchInst) * count) != hipSuccess ||
+        hipMalloc(&p2, sizeof(e2d::StepCtl) * count) != hipSuccess ||
+        hipMalloc(&p3, 64) != hipSuccess) {
+      if (p1) (void)hipFree(p1);
+      if (p2) (void)hipFree(p2);
+      e2d_batch_destroy(b);
+      return fail(E2D_ERR_ALLOC, "batch tables");
+    }
+    b->insts = static_cast<e2d::BatchInst *>(p1);
+    b->ctls = static_cast<e2d::StepCtl *>(p2);
+    b->blocks_done = static_cast<unsigned int *>(p3);
+    HIP_TRY(hipMemsetAsync(b->ctls, 0, sizeof(e2d::StepCtl) * count, b->stream));
+    HIP_TRY(hipMemsetAsync(b->blocks_done, 0, 64, b->stream));
+    int rc = batch_upload_table(b);
+    if (rc) {
+      e2d_batch_destroy(b);
+      return rc;
+    }
+    *out = b;
+    return E2D_OK;
+  });
+}
+
+int e2d_batch_destroy(e2d_batch *b) {
+  if (!b) return fail(E2D_ERR_ARG, "null e2d_batch handle");
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (e2d_run *r : b->runs) e2d_destroy(r);
+  if (b->insts) (void)hipFree(b->insts);
+  if (b->ctls) (void)hipFree(b->ctls);
+  if (b->blocks_done) (void)hipFree(b->blocks_done);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return E2D_OK;
+}
+
+int e2d_batch_init(e2d_batch *b) {
+  int rc = batch_check(b);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(b->ctls, 0, sizeof(e2d::StepCtl) * b->runs.size(), b->stream));
+  HIP_TRY(hipMemsetAsync(b->blocks_done, 0, 64, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->runs.size(); ++i) {
+    e2d_run *r = b->runs[i];
+    rc = e2d_init_problem(r);
+    if (rc) return rc;
+    HIP_TRY(e2d::launch_boundaries(r->kp, r->U, r->stream));
+    HIP_TRY(e2d::launch_boundaries(r->kp, r->U2, r->stream));
+    HIP_TRY(e2d::launch_dt_reduce(r->kp, r->U, &b->ctls[i].invdt, r->stream));
+    HIP_TRY(e2d::launch_dt_finalize(r->kp, b->ctls + i, r->stream));
+  }
+  for (e2d_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
+  return E2D_OK;
+}
+
+int e2d_batch_set_tile(e2d_batch *b, int seg_rows) {
+  int rc = batch_check(b);
+  if (rc) return rc;
+  if (seg_rows < 1 || seg_rows > 4096) return fail(E2D_ERR_ARG, "seg_rows must be in [1, 4096]");
+  b->seg_rows = seg_rows;
+  return batch_upload_table(b);
+}
+
+int e2d_batch_run(e2d_batch *b, int max_steps, int use_graph, double *elapsed_ms,
+                  int *active_left) {
+  int rc = batch_check(b);
+  if (rc) return rc;
+  if (max_steps <= 0) return fail(E2D_ERR_ARG, "max_steps must be > 0");
+  if (use_graph) {
+    rc = batch_graph(b);
+    if (rc) return rc;
+  }
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, b->stream));
+  int done = 0;
+  while (done < max_steps) {
+    if (use_graph && max_steps - done >= b->graph_steps) {
+      HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+      done += b->graph_steps;
+    } else {
+      rc = batch_launch(b);
+      if (rc) return rc;
+      done += 1;
+    }
+  }
+  HIP_TRY(hipEventRecord(e1, b->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (elapsed_ms) *elapsed_ms = ms;
+  int act = 0;
+  rc = batch_active(b, &act);
+  if (rc) return rc;
+  if (active_left) *active_left = act;
+  return E2D_OK;
+}
+
+int e2d_batch_status(e2d_batch *b, int instance, double *t, int *nStep) {
+  int rc = batch_check(b);
+  if (rc) return rc;
+  if (instance < 0 || instance >= (int)b->runs.size()) return fail(E2D_ERR_ARG, "bad instance");
+  e2d::StepCtl c{};
+  HIP_TRY(hipMemcpyAsync(&c, b->ctls + instance, sizeof(c), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (t) *t = c.t;
+  if (nStep) *nStep = c.nstep;
+  return E2D_OK;
+}
+
+int e2d_batch_copy_state(e2d_batch *b, int instance, double *host, int layout) {
+  double t;
+  int n = 0;
+  int rc = e2d_batch_status(b, instance, &t, &n);
+  if (rc) return rc;
+  return e2d_copy_state(b->runs[instance], n % 2 == 0 ? E2D_U : E2D_U2, host, layout);
+}
+
+int e2d_batch_info(e2d_batch *b, long long *cells, int *waves, int *count) {
+  if (!b) return fail(E2D_ERR_ARG, "null e2d_batch handle");
+  if (cells) *cells = b->cells;
+  if (waves) *waves = b->nwaves;
+  if (count) *count = (int)b->runs.size();
+  return E2D_OK;
+}
+
+int e2d_batch_autotune(e2d_batch *b, const int *candidates, int ncand, int steps,
+                       int *best_seg_rows, double *best_ms_per_step) {
+  int rc = batch_check(b);
+  if (rc) return rc;
+  if (!candidates || ncand <= 0 || steps <= 0) return fail(E2D_ERR_ARG, "bad argument");
+  int best = b->seg_rows;
+  double best_ms = 1e300;
+  for (int c = 0; c < ncand; ++c) {
+    rc = e2d_batch_set_tile(b, candidates[c]);
+    if (rc) return rc;
+    double ms = 0.0;
+    int act = 0;
+    rc = e2d_batch_run(b, 2, 0, &ms, &act);  // warm-up
+    if (rc) return rc;
+    rc = e2d_batch_run(b, steps, 0, &ms, &act);
+    if (rc) return rc;
+    if (ms / steps < best_ms) {
+      best_ms = ms / steps;
+      best = candidates[c];
+    }
+  }
+  rc = e2d_batch_set_tile(b, best);
+  if (rc) return rc;
+  if (best_seg_rows) *best_seg_rows = best;
+  if (best_ms_per_step) *best_ms_per_step = best_ms;
+  return E2D_OK;
+}
+
 }  // extern "C"

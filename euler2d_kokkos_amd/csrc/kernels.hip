@@ -453,152 +453,221 @@ __device__ __forceinline__ Q4T<T> shfl_dn1(const Q4T<T> &a) {
   return r;
 }
 
+// One wavefront's march over strip `strip`, segment `seg` of one grid; returns
+// the wave-local max of the next step's CFL rate when `want_rate`.
+template <int RS, class P>
+__device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in, const Grid &out,
+                                               double dt, bool want_rate, int seg_rows,
+                                               int nstrips, int strip, int seg) {
+  using T = typename P::T;
+  using Cell = typename P::Cell;
+  const SchemeT<T> &s = P::scheme(kp);
+  const int lane = threadIdx.x & (kLanes - 1);
+  T wmax = T(0.0);
+  const T dtdx = (T)(dt / kp.dx);
+  const T dtdy = (T)(dt / kp.dy);
+  const T dxr = (T)kp.dx, dyr = (T)kp.dy;
+
+  const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
+  const int i_base = 2 + kOutCols * strip;
+  const int col = i_base - 2 + lane;
+  const int j_base = 2 + seg_rows * seg;
+
+  bool flipx;
+  const int colc = min(col, kp.nx + 3);
+  const int c0 = ghost_map(colc, kp.nx, kp.bcxmin, kp.bcxmax, flipx);
+  const bool col_ghost = col < 2 || (col > kp.nx + 1 && col <= kp.nx + 3);
+  const bool col_owned = (lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1) ||
+                         (strip == 0 && lane < 2) ||
+                         (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
+  const bool out_col = lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1;
+
+  auto load_row = [&](int r) -> Cell {
+    bool flipy;
+    const int rc = min(r, kp.ny + 3);
+    const int r0 = ghost_map(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
+    Cell c = P::load(in, (long long)r0 * in.pitch + c0);
+    if (flipx) P::negate(c, IU);
+    if (flipy) P::negate(c, IV);
+    return c;
+  };
+
+  const int nrows = seg_rows + 4;
+  Cell un = load_row(j_base - 2), ua = un, ub = un;
+  Q4T<T> qm2 = {}, qm1 = {}, fymax_c = {}, fx_c = {}, fy_c = {};
+
+  for (int k = 0; k < nrows; ++k) {
+    const int r = j_base - 2 + k;
+    const Cell uc = un;
+    if (k + 1 < nrows) un = load_row(r + 1);
+
+    // ghost cells of this row owned by this wave: materialise them in both arrays
+    const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
+    const bool row_owned = (r >= j_base && r < j_base + seg_rows && r <= kp.ny + 1) ||
+                           (seg == 0 && r < 2) ||
+                           (seg == nsegs - 1 && r > kp.ny + 1 && r <= kp.ny + 3);
+    if ((col_ghost || row_ghost) && col_owned && row_owned) {
+      const long long off = (long long)r * in.pitch + col;
+      P::store(in, off, uc);
+      P::store(out, off, uc);
+    }
+
+    Q4T<T> q0;
+    P::prim(kp, uc, q0);
+
+    if (k >= 2) {
+      // row a = r - 1: slopes, source terms, y-face states
+      const Q4T<T> qpx = shfl_dn1(qm1);
+      const Q4T<T> qmx = shfl_up1(qm1);
+      Q4T<T> dqX, dqY;
+      for (int v = 0; v < 4; v++) {
+        dqX.v[v] = slope1(s, qm1.v[v], qpx.v[v], qmx.v[v]);
+        dqY.v[v] = slope1(s, qm1.v[v], q0.v[v], qm2.v[v]);
+      }
+      const SourcesT<T> src = trace_sources(s, qm1, dqX, dqY);
+      Q4T<T> fymin, fymax;
+      trace_face(s, qm1, dqY, src, dtdy, false, fymin);
+      trace_face(s, qm1, dqY, src, dtdy, true, fymax);
+      if (k >= 3) {
+        // y face between rows a-1 and a
+        T f[4];
+        riemann<RS>(s, fymax_c, fymin, IV, IU, f);
+        Q4T<T> fya;
+        for (int v = 0; v < 4; v++) fya.v[v] = f[v] * dtdy;
+        Q4T<T> fxa = {};
+        if (k <= seg_rows + 2) {
+          // x faces of row a: left state from lane-1's XMAX face
+          Q4T<T> fxmin, fxmax;
+          trace_face(s, qm1, dqX, src, dtdx, false, fxmin);
+          trace_face(s, qm1, dqX, src, dtdx, true, fxmax);
+          const Q4T<T> ql = shfl_up1(fxmax);
+          riemann<RS>(s, ql, fxmin, IU, IV, f);
+          for (int v = 0; v < 4; v++) fxa.v[v] = f[v] * dtdx;
+        }
+        if (k >= 4) {
+          // update row b = a - 1
+          const int b = r - 2;
+          const Q4T<T> fxr = shfl_dn1(fx_c);
+          const Cell unew = P::update(ub, fx_c, fxr, fy_c, fya);
+          if (out_col && b <= kp.ny + 1) {
+            P::store(out, (long long)b * out.pitch + col, unew);
+            if (want_rate) {
+              Q4T<T> qn;
+              P::prim(kp, unew, qn);
+              wmax = Ops<T>::mx(wmax, cfl_rate_q(s, qn, dxr, dyr));
+            }
+          }
+        }
+        fx_c = fxa;
+        fy_c = fya;
+      }
+      fymax_c = fymax;
+    }
+    qm2 = qm1;
+    qm1 = q0;
+    ub = ua;
+    ua = uc;
+  }
+  return wmax;
+}
+
+// Loop-state advance of one run (main.cpp:128-143): t += dt, nStep++, the
+// next dt from the reduced invDt with the tEnd clamp, and the while condition.
+__device__ __forceinline__ void advance_ctl(const KParams &kp, StepCtl *ctl) {
+  const unsigned long long bits = atomicMax(&ctl->invdt, 0ull);
+  const double invdt = __longlong_as_double((long long)bits);
+  const double t = ctl->t + ctl->dt;  // t += dt (main.cpp:143)
+  const int n = ctl->nstep + 1;
+  double dtn = kp.cfl / invdt;       // HydroRun.h:246
+  if (t + dtn > kp.tEnd) dtn = kp.tEnd - t;  // main.cpp:131-134
+  ctl->t = t;
+  ctl->nstep = n;
+  ctl->dt = dtn;
+  ctl->active = (t < kp.tEnd && n < kp.nStepmax) ? 1 : 0;
+  atomicExch(&ctl->invdt, 0ull);
+}
+
+// Folds the wave max into *invdt and reports whether this block is the last
+// of the grid to finish (all blocks' atomics are then visible).
+template <class T>
+__device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long long *invdt,
+                                                       unsigned int *blocks_done) {
+  const double wm = (double)wave_max(wmax);
+  if ((threadIdx.x & (kLanes - 1)) == 0 && wm == wm && wm > 0.0 && invdt)
+    atomicMax(invdt, dbits(wm));
+  __shared__ bool last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned int prev = atomicAdd(blocks_done, 1u);
+    last = (prev == gridDim.x - 1);
+    if (last) __threadfence();
+  }
+  __syncthreads();
+  return last;
+}
+
 template <int RS, class P>
 __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in, Grid out,
                                                             double dt_arg, StepCtl *ctl,
                                                             int seg_rows, int nstrips, int nwaves) {
   using T = typename P::T;
-  using Cell = typename P::Cell;
   if (ctl && !ctl->active) return;  // loop finished: nothing to do (uniform)
-  const SchemeT<T> &s = P::scheme(kp);
-  const int lane = threadIdx.x & (kLanes - 1);
   const int wave = blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6);
   T wmax = T(0.0);
-
-  if (wave < nwaves) {
-    const double dt = ctl ? ctl->dt : dt_arg;
-    const T dtdx = (T)(dt / kp.dx);
-    const T dtdy = (T)(dt / kp.dy);
-    const T dxr = (T)kp.dx, dyr = (T)kp.dy;
-
-    const int strip = wave % nstrips;
-    const int seg = wave / nstrips;
-    const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
-    const int i_base = 2 + kOutCols * strip;
-    const int col = i_base - 2 + lane;
-    const int j_base = 2 + seg_rows * seg;
-
-    bool flipx;
-    const int colc = min(col, kp.nx + 3);
-    const int c0 = ghost_map(colc, kp.nx, kp.bcxmin, kp.bcxmax, flipx);
-    const bool col_ghost = col < 2 || (col > kp.nx + 1 && col <= kp.nx + 3);
-    const bool col_owned = (lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1) ||
-                           (strip == 0 && lane < 2) ||
-                           (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
-    const bool out_col = lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1;
-
-    auto load_row = [&](int r) -> Cell {
-      bool flipy;
-      const int rc = min(r, kp.ny + 3);
-      const int r0 = ghost_map(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
-      Cell c = P::load(in, (long long)r0 * in.pitch + c0);
-      if (flipx) P::negate(c, IU);
-      if (flipy) P::negate(c, IV);
-      return c;
-    };
-
-    const int nrows = seg_rows + 4;
-    Cell un = load_row(j_base - 2), ua = un, ub = un;
-    Q4T<T> qm2 = {}, qm1 = {}, fymax_c = {}, fx_c = {}, fy_c = {};
-
-    for (int k = 0; k < nrows; ++k) {
-      const int r = j_base - 2 + k;
-      const Cell uc = un;
-      if (k + 1 < nrows) un = load_row(r + 1);
-
-      // ghost cells of this row owned by this wave: materialise them in both arrays
-      const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
-      const bool row_owned = (r >= j_base && r < j_base + seg_rows && r <= kp.ny + 1) ||
-                             (seg == 0 && r < 2) ||
-                             (seg == nsegs - 1 && r > kp.ny + 1 && r <= kp.ny + 3);
-      if ((col_ghost || row_ghost) && col_owned && row_owned) {
-        const long long off = (long long)r * in.pitch + col;
-        P::store(in, off, uc);
-        P::store(out, off, uc);
-      }
-
-      Q4T<T> q0;
-      P::prim(kp, uc, q0);
-
-      if (k >= 2) {
-        // row a = r - 1: slopes, source terms, y-face states
-        const Q4T<T> qpx = shfl_dn1(qm1);
-        const Q4T<T> qmx = shfl_up1(qm1);
-        Q4T<T> dqX, dqY;
-        for (int v = 0; v < 4; v++) {
-          dqX.v[v] = slope1(s, qm1.v[v], qpx.v[v], qmx.v[v]);
-          dqY.v[v] = slope1(s, qm1.v[v], q0.v[v], qm2.v[v]);
-        }
-        const SourcesT<T> src = trace_sources(s, qm1, dqX, dqY);
-        Q4T<T> fymin, fymax;
-        trace_face(s, qm1, dqY, src, dtdy, false, fymin);
-        trace_face(s, qm1, dqY, src, dtdy, true, fymax);
-        if (k >= 3) {
-          // y face between rows a-1 and a
-          T f[4];
-          riemann<RS>(s, fymax_c, fymin, IV, IU, f);
-          Q4T<T> fya;
-          for (int v = 0; v < 4; v++) fya.v[v] = f[v] * dtdy;
-          Q4T<T> fxa = {};
-          if (k <= seg_rows + 2) {
-            // x faces of row a: left state from lane-1's XMAX face
-            Q4T<T> fxmin, fxmax;
-            trace_face(s, qm1, dqX, src, dtdx, false, fxmin);
-            trace_face(s, qm1, dqX, src, dtdx, true, fxmax);
-            const Q4T<T> ql = shfl_up1(fxmax);
-            riemann<RS>(s, ql, fxmin, IU, IV, f);
-            for (int v = 0; v < 4; v++) fxa.v[v] = f[v] * dtdx;
-          }
-          if (k >= 4) {
-            // update row b = a - 1
-            const int b = r - 2;
-            const Q4T<T> fxr = shfl_dn1(fx_c);
-            const Cell unew = P::update(ub, fx_c, fxr, fy_c, fya);
-            if (out_col && b <= kp.ny + 1) {
-              P::store(out, (long long)b * out.pitch + col, unew);
-              if (ctl) {
-                Q4T<T> qn;
-                P::prim(kp, unew, qn);
-                wmax = Ops<T>::mx(wmax, cfl_rate_q(s, qn, dxr, dyr));
-              }
-            }
-          }
-          fx_c = fxa;
-          fy_c = fya;
-        }
-        fymax_c = fymax;
-      }
-      qm2 = qm1;
-      qm1 = q0;
-      ub = ua;
-      ua = uc;
-    }
-  }
-
+  if (wave < nwaves)
+    wmax = march<RS, P>(kp, in, out, ctl ? ctl->dt : dt_arg, ctl != nullptr, seg_rows, nstrips,
+                        wave % nstrips, wave / nstrips);
   if (!ctl) return;
   // run-loop epilogue: next-step invDt (exact max, order independent) and the
   // last block advances the loop state
-  const double wm = (double)wave_max(wmax);
-  if (lane == 0 && wm == wm && wm > 0.0) atomicMax(&ctl->invdt, dbits(wm));
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned int prev = atomicAdd(&ctl->blocks_done, 1u);
-    if (prev == gridDim.x - 1) {
-      __threadfence();
-      const unsigned long long bits = atomicMax(&ctl->invdt, 0ull);
-      const double invdt = __longlong_as_double((long long)bits);
-      const double t = ctl->t + ctl->dt;  // t += dt (main.cpp:143)
-      const int n = ctl->nstep + 1;
-      double dtn = kp.cfl / invdt;       // HydroRun.h:246
-      if (t + dtn > kp.tEnd) dtn = kp.tEnd - t;  // main.cpp:131-134
-      ctl->t = t;
-      ctl->nstep = n;
-      ctl->dt = dtn;
-      ctl->active = (t < kp.tEnd && n < kp.nStepmax) ? 1 : 0;
-      atomicExch(&ctl->invdt, 0ull);
-      atomicExch(&ctl->blocks_done, 0u);
+  if (reduce_rate_last_block(wmax, &ctl->invdt, &ctl->blocks_done) && threadIdx.x == 0) {
+    advance_ctl(kp, ctl);
+    atomicExch(&ctl->blocks_done, 0u);
+  }
+}
+
+// ------------------------------------------------------- batched step kernel
+//
+// One launch advances every active instance of a batch by one step, each with
+// its own dt.  Wavefront w works on the instance whose wave range contains w
+// (binary search over wave_begin), on that instance's strip / segment; its
+// in/out arrays follow the instance's own step parity.  The last block of the
+// grid advances every instance's loop state.
+template <int RS, class P>
+__global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *insts, int ninst,
+                                                            StepCtl *ctls, int nwaves,
+                                                            int seg_rows,
+                                                            unsigned int *blocks_done) {
+  using T = typename P::T;
+  const int wave = blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6);
+  T wmax = T(0.0);
+  int inst = -1;
+  if (wave < nwaves) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {  // last instance with wave_begin <= wave
+      const int mid = (lo + hi + 1) >> 1;
+      if (insts[mid].wave_begin <= wave)
+        lo = mid;
+      else
+        hi = mid - 1;
     }
+    inst = lo;
+    const BatchInst &bi = insts[inst];
+    StepCtl *ctl = ctls + inst;
+    if (ctl->active) {
+      const int w = wave - bi.wave_begin;
+      const bool even = (ctl->nstep & 1) == 0;
+      wmax = march<RS, P>(bi.kp, even ? bi.U : bi.U2, even ? bi.U2 : bi.U, ctl->dt, true,
+                          seg_rows, bi.nstrips, w % bi.nstrips, w / bi.nstrips);
+    }
+  }
+  const bool last = reduce_rate_last_block(wmax, inst >= 0 ? &ctls[inst].invdt : nullptr,
+                                           blocks_done);
+  if (last) {
+    for (int i = threadIdx.x; i < ninst; i += blockDim.x)
+      if (ctls[i].active) advance_ctl(insts[i].kp, ctls + i);
+    if (threadIdx.x == 0) atomicExch(blocks_done, 0u);
   }
 }
 
@@ -799,6 +868,32 @@ hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, St
     launch_fused_p<PSplit>(kp, in, out, dt, ctl, seg_rows, st);
   else
     launch_fused_p<P64>(kp, in, out, dt, ctl, seg_rows, st);
+  return hipGetLastError();
+}
+
+int batch_waves(const KParams &kp, int seg_rows, int *nstrips) {
+  const int ns = (kp.nx + kOutCols - 1) / kOutCols;
+  if (nstrips) *nstrips = ns;
+  return ns * ((kp.ny + seg_rows - 1) / seg_rows);
+}
+
+hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
+                             StepCtl *ctls_dev, int nwaves, int seg_rows,
+                             unsigned int *blocks_done, hipStream_t st) {
+  const int blocks = (nwaves + kBlock / kLanes - 1) / (kBlock / kLanes);
+#define E2D_BATCH_LAUNCH(RS, P)                                                              \
+  hipLaunchKernelGGL((step_batch_kernel<RS, P>), dim3(blocks), dim3(kBlock), 0, st, insts_dev, \
+                     ninst, ctls_dev, nwaves, seg_rows, blocks_done)
+  if (kp0.prec == E2D_PRECISION_FP32_SPLIT) {
+    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, PSplit);
+    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, PSplit);
+    else E2D_BATCH_LAUNCH(0, PSplit);
+  } else {
+    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, P64);
+    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, P64);
+    else E2D_BATCH_LAUNCH(0, P64);
+  }
+#undef E2D_BATCH_LAUNCH
   return hipGetLastError();
 }
 

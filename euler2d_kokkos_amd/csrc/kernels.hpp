@@ -48,6 +48,15 @@ struct StepCtl {
   int pad;
 };
 
+// One instance of a batch as the batched step kernel sees it.
+struct BatchInst {
+  KParams kp;
+  Grid U, U2;
+  int wave_begin;  // first global wavefront index of this instance
+  int nstrips;
+  int pad[2];
+};
+
 // fused step geometry: one wave of 64 lanes covers 60 output columns (plus a
 // 2-column halo on each side) and marches over `seg_rows` output rows
 constexpr int kLanes = 64;
@@ -77,6 +86,12 @@ hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, St
                              int seg_rows, hipStream_t st);
 hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st);
 int sums_partials_needed(const KParams &kp);
+// wavefronts the fused step launches for one grid (and its strip count)
+int batch_waves(const KParams &kp, int seg_rows, int *nstrips);
+// one step of every active instance; kp0 gives the precision and solver of all
+hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
+                             StepCtl *ctls_dev, int nwaves, int seg_rows,
+                             unsigned int *blocks_done, hipStream_t st);
 // whole-array copy (all planes, ghosts and padding included)
 hipError_t launch_copy(const KParams &kp, Grid src, Grid dst, hipStream_t st);
 

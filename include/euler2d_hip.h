@@ -195,6 +195,36 @@ int e2d_synchronize(e2d_run *run);
 int e2d_time_loop(e2d_run *run, int nsteps, double *elapsed_ms, int *executed, double *t,
                   int *nStep);
 
+/* ------------------------------------------------------------------ batch */
+/* Many independent problems advanced together (no reference counterpart: the
+ * reference runs one grid per process, src/main.cpp:86).  Every instance keeps
+ * the HydroRun semantics above -- its own grid size, problem, dt, t, nStep --
+ * and all share the precision, the Riemann solver and the step kernel: one
+ * launch advances every active instance by one step, and runs of steps are
+ * replayed from a captured hipGraph. */
+typedef struct e2d_batch e2d_batch;
+int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_batch **out);
+int e2d_batch_destroy(e2d_batch *batch);
+/* Init<Problem> of every instance, boundary fill, and the startup dt of each
+ * instance (main.cpp:87-91); resets t and nStep to 0 */
+int e2d_batch_init(e2d_batch *batch);
+int e2d_batch_set_tile(e2d_batch *batch, int seg_rows);
+/* Runs up to max_steps batched steps (instances whose loop has ended stay put).
+ * use_graph != 0 replays captured hipGraphs of the step launch.  Returns the
+ * device time in milliseconds and the number of instances still active. */
+int e2d_batch_run(e2d_batch *batch, int max_steps, int use_graph, double *elapsed_ms,
+                  int *active_left);
+int e2d_batch_status(e2d_batch *batch, int instance, double *t, int *nStep);
+/* current state (U if the instance's nStep is even, else U2) of one instance */
+int e2d_batch_copy_state(e2d_batch *batch, int instance, double *host, int layout);
+/* total interior cells, wavefronts per step, instance count */
+int e2d_batch_info(e2d_batch *batch, long long *cells, int *waves, int *count);
+/* Times each candidate seg_rows over `steps` batched steps (restoring the
+ * state afterwards is the caller's business: call e2d_batch_init) and keeps the
+ * fastest; returns it. */
+int e2d_batch_autotune(e2d_batch *batch, const int *candidates, int ncand, int steps,
+                       int *best_seg_rows, double *best_ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,73 @@
+"""Batched driver on the GPU: every instance of a batch must equal the same
+problem run alone (bit for bit: the batched kernel runs the same march per
+wavefront), in FP64 (and hence the CPU oracle) and in split precision."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64, HydroParams, HydroRun
+from euler2d_kokkos_amd.batch import HydroBatch, make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_instances():
+    return [
+        make_params(nx=64, ny=96, problemType=0, cfl=0.8, nStepmax=30, tEnd=1e9),
+        make_params(nx=128, ny=64, problemType=1, cfl=0.7, nStepmax=25, tEnd=1e9,
+                    blast_radius=0.15, blast_pressure_in=50.0),
+        make_params(nx=96, ny=96, problemType=0, cfl=0.8, nStepmax=1000, tEnd=0.02,
+                    implode_pressure_in=1.7),
+        make_params(nx=61, ny=130, problemType=1, cfl=0.6, nStepmax=40, tEnd=1e9,
+                    boundary_type_xmin=3, boundary_type_xmax=3, blast_center_y=0.3),
+        make_params(nx=200, ny=33, problemType=3, cfl=0.8, nStepmax=20, tEnd=1e9,
+                    boundary_type_ymin=2, boundary_type_ymax=2),
+        make_params(nx=64, ny=64, problemType=2, cfl=0.8, nStepmax=35, tEnd=1e9,
+                    boundary_type_xmin=2, boundary_type_xmax=2, boundary_type_ymin=2,
+                    boundary_type_ymax=2),
+    ]
+
+
+@pytest.mark.parametrize("precision", [E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT],
+                         ids=["fp64", "split"])
+@pytest.mark.parametrize("use_graph", [False, True], ids=["launches", "graph"])
+def test_batch_equals_single_runs(gpu_available, precision, use_graph):
+    ps = mixed_instances()
+    batch = HydroBatch(ps, precision=precision)
+    batch.set_tile(12)
+    ms, active = batch.run(64, use_graph=use_graph)
+    assert active == 0
+    for i, p in enumerate(ps):
+        run = HydroRun(HydroParams(p.copy()), precision=precision)
+        run.set_tile(7)  # results do not depend on the tile
+        run.run()
+        t, n = batch.status(i)
+        assert (t, n) == (run.t, run.nStep), i
+        np.testing.assert_array_equal(batch.state(i), run.get_state(run.current()),
+                                      err_msg=f"instance {i}")
+
+
+def test_batch_fp64_equals_oracle(gpu_available):
+    ps = mixed_instances()[:3]
+    batch = HydroBatch(ps, precision=E2D_PRECISION_FP64)
+    batch.run(40)
+    for i, p in enumerate(ps):
+        ora = O.OracleRun(p)
+        ora.run()
+        assert batch.status(i) == (ora.t, ora.nStep)
+        np.testing.assert_array_equal(batch.state(i), ora.current())
+
+
+def test_batch_autotune_and_reinit(gpu_available):
+    ps = mixed_instances()
+    batch = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
+    best, ms = batch.autotune(candidates=(6, 12, 24), steps=3)
+    assert best in (6, 12, 24) and ms > 0
+    batch.init()
+    batch.run(10, use_graph=False)
+    a = [batch.state(i) for i in range(len(ps))]
+    batch.init()
+    batch.set_tile(9)
+    batch.run(10, use_graph=True)
+    for i in range(len(ps)):
+        np.testing.assert_array_equal(batch.state(i), a[i])
