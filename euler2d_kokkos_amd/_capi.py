@@ -113,6 +113,8 @@ _PROTOTYPES = {
     "e2d_set_schedule": (C.c_int, [C.c_void_p, C.c_int]),
     "e2d_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
     "e2d_get_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "e2d_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int,
+                               C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "e2d_synchronize": (C.c_int, [C.c_void_p]),
     "e2d_time_loop": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
                                 C.POINTER(C.c_double), C.POINTER(C.c_int)]),

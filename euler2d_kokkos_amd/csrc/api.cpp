@@ -714,6 +714,65 @@ int e2d_get_tile(e2d_run *r, int *seg_rows) {
   return E2D_OK;
 }
 
+int e2d_autotune(e2d_run *r, const int *candidates, int ncand, int steps, int *best_seg_rows,
+                 double *best_ms_per_step) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (!candidates || ncand <= 0 || steps <= 0) return fail(E2D_ERR_ARG, "bad argument");
+  for (int c = 0; c < ncand; ++c)
+    if (candidates[c] < 1 || candidates[c] > 4096) return fail(E2D_ERR_ARG, "bad candidate");
+  const int schedule = r->schedule;
+  r->schedule = E2D_SCHEDULE_FUSED;
+  // snapshot of both arrays; the tuning runs advance a throw-away copy of the loop
+  Grid s1{}, s2{};
+  rc = alloc_grid(r, s1, r->precision);
+  if (!rc) rc = alloc_grid(r, s2, r->precision);
+  if (rc) return rc;
+  HIP_TRY(e2d::launch_copy(r->kp, r->U, s1, r->stream));
+  HIP_TRY(e2d::launch_copy(r->kp, r->U2, s2, r->stream));
+  const double timers1 = r->timers[1];
+  const int tile0 = r->seg_rows;
+  int best = tile0;
+  double best_ms = 1e300;
+  for (int c = 0; c < ncand && !rc; ++c) {
+    r->seg_rows = candidates[c];
+    double t = 0.0, ms = 0.0;
+    int n = 0, ex = 0;
+    rc = run_loop_impl(r, 2, &t, &n, nullptr, nullptr);  // warm-up
+    if (!rc) {
+      t = 0.0;
+      n = 0;
+      rc = run_loop_impl(r, steps, &t, &n, &ms, &ex);
+    }
+    if (!rc && ex > 0 && ms / ex < best_ms) {
+      best_ms = ms / ex;
+      best = candidates[c];
+    }
+    HIP_TRY(e2d::launch_copy(r->kp, s1, r->U, r->stream));
+    HIP_TRY(e2d::launch_copy(r->kp, s2, r->U2, r->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  // release the snapshot arrays
+  for (void *ptr : {s1.p, s2.p}) {
+    for (auto it = r->allocs.begin(); it != r->allocs.end(); ++it)
+      if (*it == ptr) {
+        r->allocs.erase(it);
+        break;
+      }
+    (void)hipFree(ptr);
+  }
+  r->timers[1] = timers1;
+  r->schedule = schedule;
+  if (rc) {
+    r->seg_rows = tile0;
+    return rc;
+  }
+  r->seg_rows = best;
+  if (best_seg_rows) *best_seg_rows = best;
+  if (best_ms_per_step) *best_ms_per_step = best_ms;
+  return E2D_OK;
+}
+
 int e2d_synchronize(e2d_run *r) {
   int rc = check_run(r);
   if (rc) return rc;

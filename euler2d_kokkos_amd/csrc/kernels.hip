@@ -453,20 +453,126 @@ __device__ __forceinline__ Q4T<T> shfl_dn1(const Q4T<T> &a) {
   return r;
 }
 
+// Stages of one march iteration over loaded row r (row a = r-1, b = r-2).
+enum : int {
+  kSlopes = 1,  // slopes, source terms and y-face states of row a
+  kYFace = 2,   // y-face flux between rows a-1 and a
+  kXFace = 4,   // x-face fluxes of row a
+  kUpdate = 8,  // update of row b (+ its next-step CFL rate)
+  kAll = 15
+};
+
+// Everything one wavefront carries while marching up its strip.
+template <class P>
+struct MarchState {
+  typename P::Cell un, ua, ub;  // rows r+1 (prefetched), r-1, r-2
+  Q4T<typename P::T> qm2, qm1, fymax_c, fx_c, fy_c;
+  typename P::T wmax;
+};
+
+// One iteration over loaded row r.  STAGES is a compile-time set, so the
+// steady-state iteration is one branch-free basic block in which the compiler
+// can interleave the independent x-face and y-face Riemann chains.
+template <int RS, class P, int STAGES, bool PREFETCH>
+__device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, const Grid &out,
+                                          MarchState<P> &m, int r, int c0, bool flipx,
+                                          typename P::T dtdx, typename P::T dtdy,
+                                          bool ghost_col, bool out_col, int col, int row_lo,
+                                          int row_hi, bool want_rate) {
+  using T = typename P::T;
+  using Cell = typename P::Cell;
+  const SchemeT<T> &s = P::scheme(kp);
+  const Cell uc = m.un;
+  if constexpr (PREFETCH) {
+    bool flipy;
+    const int rc = min(r + 1, kp.ny + 3);
+    const int r0 = ghost_map(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
+    m.un = P::load(in, (long long)r0 * in.pitch + c0);
+    if (flipx) P::negate(m.un, IU);
+    if (flipy) P::negate(m.un, IV);
+  }
+  Q4T<T> q0;
+  P::prim(kp, uc, q0);
+
+  Q4T<T> fya = {}, fxa = {}, fymax = {};
+  Cell unew;
+  if constexpr ((STAGES & kSlopes) != 0) {
+    const Q4T<T> qpx = shfl_dn1(m.qm1);
+    const Q4T<T> qmx = shfl_up1(m.qm1);
+    Q4T<T> dqX, dqY;
+    for (int v = 0; v < 4; v++) {
+      dqX.v[v] = slope1(s, m.qm1.v[v], qpx.v[v], qmx.v[v]);
+      dqY.v[v] = slope1(s, m.qm1.v[v], q0.v[v], m.qm2.v[v]);
+    }
+    const SourcesT<T> src = trace_sources(s, m.qm1, dqX, dqY);
+    Q4T<T> fymin;
+    trace_face(s, m.qm1, dqY, src, dtdy, false, fymin);
+    trace_face(s, m.qm1, dqY, src, dtdy, true, fymax);
+    if constexpr ((STAGES & kYFace) != 0) {
+      T f[4];
+      riemann<RS>(s, m.fymax_c, fymin, IV, IU, f);
+      for (int v = 0; v < 4; v++) fya.v[v] = f[v] * dtdy;
+    }
+    if constexpr ((STAGES & kXFace) != 0) {
+      Q4T<T> fxmin, fxmax;
+      trace_face(s, m.qm1, dqX, src, dtdx, false, fxmin);
+      trace_face(s, m.qm1, dqX, src, dtdx, true, fxmax);
+      const Q4T<T> ql = shfl_up1(fxmax);
+      T f[4];
+      riemann<RS>(s, ql, fxmin, IU, IV, f);
+      for (int v = 0; v < 4; v++) fxa.v[v] = f[v] * dtdx;
+    }
+    if constexpr ((STAGES & kUpdate) != 0) {
+      const Q4T<T> fxr = shfl_dn1(m.fx_c);
+      unew = P::update(m.ub, m.fx_c, fxr, m.fy_c, fya);
+    }
+  }
+
+  // stores last, so that the arithmetic above stays in one basic block
+  // ghost cells of row r owned by this wave: materialise them in both arrays
+  const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
+  if ((ghost_col || row_ghost) && r >= row_lo && r <= row_hi) {
+    const long long off = (long long)r * in.pitch + col;
+    P::store(in, off, uc);
+    P::store(out, off, uc);
+  }
+  if constexpr ((STAGES & kUpdate) != 0) {
+    const int b = r - 2;
+    if (out_col && b <= kp.ny + 1) {
+      P::store(out, (long long)b * out.pitch + col, unew);
+      if (want_rate) {
+        Q4T<T> qn;
+        P::prim(kp, unew, qn);
+        m.wmax = Ops<T>::mx(m.wmax, cfl_rate_q(s, qn, (T)kp.dx, (T)kp.dy));
+      }
+    }
+  }
+  if constexpr ((STAGES & kYFace) != 0) {
+    m.fx_c = fxa;
+    m.fy_c = fya;
+  }
+  if constexpr ((STAGES & kSlopes) != 0) m.fymax_c = fymax;
+  m.qm2 = m.qm1;
+  m.qm1 = q0;
+  m.ub = m.ua;
+  m.ua = uc;
+}
+
 // One wavefront's march over strip `strip`, segment `seg` of one grid; returns
 // the wave-local max of the next step's CFL rate when `want_rate`.
+//
+// Iteration k loads row r = j_base-2+k: k = 0,1 only convert; k = 2 adds the
+// slopes of row j_base-1; k = 3 the first y and x faces; k = 4 .. seg_rows+2
+// is the steady state (all stages, branch-free); k = seg_rows+3 finishes the
+// last y face and update.
 template <int RS, class P>
 __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in, const Grid &out,
                                                double dt, bool want_rate, int seg_rows,
                                                int nstrips, int strip, int seg) {
   using T = typename P::T;
-  using Cell = typename P::Cell;
-  const SchemeT<T> &s = P::scheme(kp);
   const int lane = threadIdx.x & (kLanes - 1);
-  T wmax = T(0.0);
   const T dtdx = (T)(dt / kp.dx);
   const T dtdy = (T)(dt / kp.dy);
-  const T dxr = (T)kp.dx, dyr = (T)kp.dy;
 
   const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
   const int i_base = 2 + kOutCols * strip;
@@ -481,94 +587,39 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
                          (strip == 0 && lane < 2) ||
                          (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
   const bool out_col = lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1;
+  // Ghost cells this wave writes: owned column, owned row, and a ghost column or
+  // ghost row.  The owned rows of a segment are contiguous: [row_lo, row_hi].
+  const int row_lo = seg == 0 ? 0 : j_base;
+  const int row_hi = seg == nsegs - 1 ? kp.ny + 3 : min(j_base + seg_rows - 1, kp.ny + 1);
+  const bool ghost_col = col_ghost && col_owned;
+  const int wlo = col_owned ? row_lo : 1;   // empty range for columns not owned
+  const int whi = col_owned ? row_hi : -1;
 
-  auto load_row = [&](int r) -> Cell {
+  MarchState<P> m;
+  {
     bool flipy;
-    const int rc = min(r, kp.ny + 3);
-    const int r0 = ghost_map(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
-    Cell c = P::load(in, (long long)r0 * in.pitch + c0);
-    if (flipx) P::negate(c, IU);
-    if (flipy) P::negate(c, IV);
-    return c;
-  };
-
-  const int nrows = seg_rows + 4;
-  Cell un = load_row(j_base - 2), ua = un, ub = un;
-  Q4T<T> qm2 = {}, qm1 = {}, fymax_c = {}, fx_c = {}, fy_c = {};
-
-  for (int k = 0; k < nrows; ++k) {
-    const int r = j_base - 2 + k;
-    const Cell uc = un;
-    if (k + 1 < nrows) un = load_row(r + 1);
-
-    // ghost cells of this row owned by this wave: materialise them in both arrays
-    const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
-    const bool row_owned = (r >= j_base && r < j_base + seg_rows && r <= kp.ny + 1) ||
-                           (seg == 0 && r < 2) ||
-                           (seg == nsegs - 1 && r > kp.ny + 1 && r <= kp.ny + 3);
-    if ((col_ghost || row_ghost) && col_owned && row_owned) {
-      const long long off = (long long)r * in.pitch + col;
-      P::store(in, off, uc);
-      P::store(out, off, uc);
-    }
-
-    Q4T<T> q0;
-    P::prim(kp, uc, q0);
-
-    if (k >= 2) {
-      // row a = r - 1: slopes, source terms, y-face states
-      const Q4T<T> qpx = shfl_dn1(qm1);
-      const Q4T<T> qmx = shfl_up1(qm1);
-      Q4T<T> dqX, dqY;
-      for (int v = 0; v < 4; v++) {
-        dqX.v[v] = slope1(s, qm1.v[v], qpx.v[v], qmx.v[v]);
-        dqY.v[v] = slope1(s, qm1.v[v], q0.v[v], qm2.v[v]);
-      }
-      const SourcesT<T> src = trace_sources(s, qm1, dqX, dqY);
-      Q4T<T> fymin, fymax;
-      trace_face(s, qm1, dqY, src, dtdy, false, fymin);
-      trace_face(s, qm1, dqY, src, dtdy, true, fymax);
-      if (k >= 3) {
-        // y face between rows a-1 and a
-        T f[4];
-        riemann<RS>(s, fymax_c, fymin, IV, IU, f);
-        Q4T<T> fya;
-        for (int v = 0; v < 4; v++) fya.v[v] = f[v] * dtdy;
-        Q4T<T> fxa = {};
-        if (k <= seg_rows + 2) {
-          // x faces of row a: left state from lane-1's XMAX face
-          Q4T<T> fxmin, fxmax;
-          trace_face(s, qm1, dqX, src, dtdx, false, fxmin);
-          trace_face(s, qm1, dqX, src, dtdx, true, fxmax);
-          const Q4T<T> ql = shfl_up1(fxmax);
-          riemann<RS>(s, ql, fxmin, IU, IV, f);
-          for (int v = 0; v < 4; v++) fxa.v[v] = f[v] * dtdx;
-        }
-        if (k >= 4) {
-          // update row b = a - 1
-          const int b = r - 2;
-          const Q4T<T> fxr = shfl_dn1(fx_c);
-          const Cell unew = P::update(ub, fx_c, fxr, fy_c, fya);
-          if (out_col && b <= kp.ny + 1) {
-            P::store(out, (long long)b * out.pitch + col, unew);
-            if (want_rate) {
-              Q4T<T> qn;
-              P::prim(kp, unew, qn);
-              wmax = Ops<T>::mx(wmax, cfl_rate_q(s, qn, dxr, dyr));
-            }
-          }
-        }
-        fx_c = fxa;
-        fy_c = fya;
-      }
-      fymax_c = fymax;
-    }
-    qm2 = qm1;
-    qm1 = q0;
-    ub = ua;
-    ua = uc;
+    const int r0 = ghost_map(min(j_base - 2, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
+    m.un = P::load(in, (long long)r0 * in.pitch + c0);
+    if (flipx) P::negate(m.un, IU);
+    if (flipy) P::negate(m.un, IV);
   }
-  return wmax;
+  m.ua = m.un;
+  m.ub = m.un;
+  m.qm2 = m.qm1 = m.fymax_c = m.fx_c = m.fy_c = Q4T<T>{};
+  m.wmax = T(0.0);
+
+  const int r0 = j_base - 2;
+#define E2D_ROW(ST, PF, k)                                                                 \
+  march_row<RS, P, ST, PF>(kp, in, out, m, r0 + (k), c0, flipx, dtdx, dtdy, ghost_col, out_col, \
+                           col, wlo, whi, want_rate)
+  E2D_ROW(0, true, 0);
+  E2D_ROW(0, true, 1);
+  E2D_ROW(kSlopes, true, 2);
+  E2D_ROW(kSlopes | kYFace | kXFace, true, 3);
+  for (int k = 4; k <= seg_rows + 2; ++k) E2D_ROW(kAll, true, k);
+  E2D_ROW(kSlopes | kYFace | kUpdate, false, seg_rows + 3);
+#undef E2D_ROW
+  return m.wmax;
 }
 
 // Loop-state advance of one run (main.cpp:128-143): t += dt, nStep++, the
@@ -652,7 +703,7 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *ins
       else
         hi = mid - 1;
     }
-    inst = lo;
+    inst = __builtin_amdgcn_readfirstlane(lo);  // wave-uniform: scalar loads of the table
     const BatchInst &bi = insts[inst];
     StepCtl *ctl = ctls + inst;
     if (ctl->active) {

@@ -188,6 +188,10 @@ int e2d_set_schedule(e2d_run *run, int schedule);
  * wavefront covers 60 output columns.  Results do not depend on it. */
 int e2d_set_tile(e2d_run *run, int seg_rows);
 int e2d_get_tile(e2d_run *run, int *seg_rows);
+/* Times `steps` loop iterations for each candidate seg_rows on the current
+ * state (restored afterwards, t and nStep untouched) and keeps the fastest. */
+int e2d_autotune(e2d_run *run, const int *candidates, int ncand, int steps, int *best_seg_rows,
+                 double *best_ms_per_step);
 int e2d_synchronize(e2d_run *run);
 /* Device-side timing of `nsteps` loop iterations (e2d_run_loop semantics) with
  * HIP events on the handle's stream; returns the elapsed milliseconds and the

@@ -144,3 +144,22 @@ def test_conserved_sums_reflecting(gpu_available):
     s1 = run.conserved_sums(run.current())
     assert abs(s1[0] - s0[0]) <= 1e-12 * abs(s0[0])
     assert abs(s1[1] - s0[1]) <= 1e-12 * abs(s0[1])
+
+
+def test_autotune_restores_state(gpu_available):
+    p = _params("blast48x72")
+    run = HydroRun(HydroParams(p.copy()))
+    run.run(5)
+    before = [run.get_state(w) for w in (E2D_U, E2D_U2)]
+    t, n = run.t, run.nStep
+    best, ms = run.autotune(candidates=(4, 9, 16), steps=3)
+    assert best in (4, 9, 16) and ms > 0 and run.get_tile() == best
+    assert (run.t, run.nStep) == (t, n)
+    for w, b in zip((E2D_U, E2D_U2), before):
+        np.testing.assert_array_equal(run.get_state(w), b)
+    # the tuned run continues exactly like an untuned one
+    ora = O.OracleRun(p)
+    ora.run(5)
+    run.run()
+    ora.run()
+    _compare(run, ora, "autotuned")
