@@ -28,6 +28,7 @@ E2D_LAYOUT_INTERIOR = 2
 
 E2D_PRECISION_FP64 = 0
 E2D_PRECISION_FP32_SPLIT = 1
+E2D_PRECISION_FP32 = 2
 
 E2D_SCHEDULE_FUSED = 0
 E2D_SCHEDULE_PASSES = 1

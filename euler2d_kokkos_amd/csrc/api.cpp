@@ -294,9 +294,10 @@ int download_planes(e2d_run *r, const Grid &g, std::vector<double> &out) {
     HIP_TRY(hipStreamSynchronize(r->stream));
     return E2D_OK;
   }
-  std::vector<float> tmp(8 * cells);
+  const int nk = e2d::grid_planes(r->precision);
+  std::vector<float> tmp(nk * cells);
   const float *src = static_cast<const float *>(g.p);
-  for (int k = 0; k < 8; k++)
+  for (int k = 0; k < nk; k++)
     HIP_TRY(hipMemcpy2DAsync(tmp.data() + k * cells, (size_t)isz * sizeof(float),
                              src + k * g.plane, (size_t)g.pitch * sizeof(float),
                              (size_t)isz * sizeof(float), (size_t)jsz, hipMemcpyDeviceToHost,
@@ -304,7 +305,9 @@ int download_planes(e2d_run *r, const Grid &g, std::vector<double> &out) {
   HIP_TRY(hipStreamSynchronize(r->stream));
   for (int v = 0; v < 4; v++)
     for (size_t c = 0; c < cells; c++)
-      out[v * cells + c] = (double)tmp[(2 * v) * cells + c] + (double)tmp[(2 * v + 1) * cells + c];
+      out[v * cells + c] = nk == 8 ? (double)tmp[(2 * v) * cells + c] +
+                                         (double)tmp[(2 * v + 1) * cells + c]
+                                   : (double)tmp[v * cells + c];
   return E2D_OK;
 }
 
@@ -322,16 +325,21 @@ int upload_planes(e2d_run *r, const Grid &g, const std::vector<double> &in) {
     HIP_TRY(hipStreamSynchronize(r->stream));
     return E2D_OK;
   }
-  std::vector<float> tmp(8 * cells);
+  const int nk = e2d::grid_planes(r->precision);
+  std::vector<float> tmp(nk * cells);
   for (int v = 0; v < 4; v++)
     for (size_t c = 0; c < cells; c++) {
       const double x = in[v * cells + c];
       const float hi = (float)x;
-      tmp[(2 * v) * cells + c] = hi;
-      tmp[(2 * v + 1) * cells + c] = (float)(x - (double)hi);
+      if (nk == 8) {
+        tmp[(2 * v) * cells + c] = hi;
+        tmp[(2 * v + 1) * cells + c] = (float)(x - (double)hi);
+      } else {
+        tmp[v * cells + c] = hi;
+      }
     }
   float *dst = static_cast<float *>(g.p);
-  for (int k = 0; k < 8; k++)
+  for (int k = 0; k < nk; k++)
     HIP_TRY(hipMemcpy2DAsync(dst + k * g.plane, (size_t)g.pitch * sizeof(float),
                              tmp.data() + k * cells, (size_t)isz * sizeof(float),
                              (size_t)isz * sizeof(float), (size_t)jsz, hipMemcpyHostToDevice,
@@ -509,8 +517,9 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
   *out = nullptr;
   int rc = e2d_params_validate(p);
   if (rc) return rc;
-  if (precision != E2D_PRECISION_FP64 && precision != E2D_PRECISION_FP32_SPLIT)
-    return fail(E2D_ERR_ARG, "precision must be E2D_PRECISION_FP64 or E2D_PRECISION_FP32_SPLIT");
+  if (precision != E2D_PRECISION_FP64 && precision != E2D_PRECISION_FP32_SPLIT &&
+      precision != E2D_PRECISION_FP32)
+    return fail(E2D_ERR_ARG, "precision must be one of E2D_PRECISION_FP64 / _FP32_SPLIT / _FP32");
   return guarded([&]() -> int {
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);

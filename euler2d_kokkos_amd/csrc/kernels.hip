@@ -144,6 +144,49 @@ struct PSplit {
   }
 };
 
+// Plain FP32 (the reference built with USE_DOUBLE=OFF, CMakeLists.txt:48-51):
+// four float planes, float arithmetic, the update in float.  A comparator for
+// the split path, not a drop-in.
+struct P32 {
+  using T = float;
+  struct Cell {
+    float u[4];
+  };
+  static __device__ __forceinline__ Cell load(const Grid &g, long long off) {
+    const float *p = static_cast<const float *>(g.p);
+    Cell c;
+    for (int v = 0; v < 4; v++) c.u[v] = p[v * g.plane + off];
+    return c;
+  }
+  static __device__ __forceinline__ void store(const Grid &g, long long off, const Cell &c) {
+    float *p = static_cast<float *>(g.p);
+    for (int v = 0; v < 4; v++) p[v * g.plane + off] = c.u[v];
+  }
+  static __device__ __forceinline__ void negate(Cell &c, int v) { c.u[v] = -c.u[v]; }
+  static __device__ __forceinline__ Cell from_double(double d, double e, double mx, double my) {
+    return Cell{{(float)d, (float)e, (float)mx, (float)my}};
+  }
+  static __device__ __forceinline__ double value(const Cell &c, int v) { return (double)c.u[v]; }
+  static __device__ __forceinline__ const SchemeT<T> &scheme(const KParams &kp) { return kp.sf; }
+  static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
+    prim_no_c(kp.sf, c.u[0], c.u[1], c.u[2], c.u[3], q);
+  }
+  static __device__ __forceinline__ Cell update(const Cell &c, const Q4T<T> &fxl,
+                                                const Q4T<T> &fxr, const Q4T<T> &fyb,
+                                                const Q4T<T> &fyt) {
+    Cell o;
+    for (int v = 0; v < 4; v++) {
+      float u = c.u[v];
+      u += fxl.v[v];
+      u -= fxr.v[v];
+      u += fyb.v[v];
+      u -= fyt.v[v];
+      o.u[v] = u;
+    }
+    return o;
+  }
+};
+
 __device__ __forceinline__ double &at(const Grid &g, int v, int i, int j) {
   return static_cast<double *>(g.p)[v * g.plane + (long long)j * g.pitch + i];
 }
@@ -155,6 +198,7 @@ __device__ __forceinline__ double &at(const Grid &g, int v, int i, int j) {
 // fills x faces then y faces, so a corner ghost is the y rule applied to the x
 // rule: both maps compose, and the sign flips (exact) commute.
 __device__ __forceinline__ int ghost_map(int i, int n, int bclo, int bchi, bool &flip) {
+  // for a row index this is uniform control flow on the scalar unit
   flip = false;
   if (i < 2) {
     if (bclo == 1) {
@@ -439,17 +483,38 @@ __global__ void update_kernel(KParams kp, Grid U, Grid Fx, Grid Fy) {
 // block, folds the next step's CFL rate of every updated cell into a wave max,
 // and the last block to finish advances t, nStep and the next dt.
 
+// Whole-wavefront lane shifts by one with DPP (GFX9 wave_shr:1 / wave_shl:1):
+// a full-rate VALU move instead of an LDS-routed permute.  Lane 0 (shift up)
+// and lane 63 (shift down) keep their own value; the march never uses them.
+constexpr int kDppWaveShr1 = 0x138;  // lane i reads lane i-1
+constexpr int kDppWaveShl1 = 0x130;  // lane i reads lane i+1
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float x) {
+  const int v = __float_as_int(x);
+  return __int_as_float(__builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
+}
+
 template <class T>
 __device__ __forceinline__ Q4T<T> shfl_up1(const Q4T<T> &a) {
   Q4T<T> r;
-  for (int v = 0; v < 4; v++) r.v[v] = __shfl_up(a.v[v], 1);
+  for (int v = 0; v < 4; v++) r.v[v] = dpp_shift<kDppWaveShr1>(a.v[v]);
   return r;
 }
 
 template <class T>
 __device__ __forceinline__ Q4T<T> shfl_dn1(const Q4T<T> &a) {
   Q4T<T> r;
-  for (int v = 0; v < 4; v++) r.v[v] = __shfl_down(a.v[v], 1);
+  for (int v = 0; v < 4; v++) r.v[v] = dpp_shift<kDppWaveShl1>(a.v[v]);
   return r;
 }
 
@@ -473,12 +538,12 @@ struct MarchState {
 // One iteration over loaded row r.  STAGES is a compile-time set, so the
 // steady-state iteration is one branch-free basic block in which the compiler
 // can interleave the independent x-face and y-face Riemann chains.
-template <int RS, class P, int STAGES, bool PREFETCH>
+template <int RS, class P, int STAGES, bool PREFETCH, bool RATE>
 __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, const Grid &out,
                                           MarchState<P> &m, int r, int c0, bool flipx,
                                           typename P::T dtdx, typename P::T dtdy,
                                           bool ghost_col, bool out_col, int col, int row_lo,
-                                          int row_hi, bool want_rate) {
+                                          int row_hi) {
   using T = typename P::T;
   using Cell = typename P::Cell;
   const SchemeT<T> &s = P::scheme(kp);
@@ -540,7 +605,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     const int b = r - 2;
     if (out_col && b <= kp.ny + 1) {
       P::store(out, (long long)b * out.pitch + col, unew);
-      if (want_rate) {
+      if constexpr (RATE) {  // next-step CFL rate of the new state
         Q4T<T> qn;
         P::prim(kp, unew, qn);
         m.wmax = Ops<T>::mx(m.wmax, cfl_rate_q(s, qn, (T)kp.dx, (T)kp.dy));
@@ -565,10 +630,10 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
 // slopes of row j_base-1; k = 3 the first y and x faces; k = 4 .. seg_rows+2
 // is the steady state (all stages, branch-free); k = seg_rows+3 finishes the
 // last y face and update.
-template <int RS, class P>
+template <int RS, class P, bool RATE>
 __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in, const Grid &out,
-                                               double dt, bool want_rate, int seg_rows,
-                                               int nstrips, int strip, int seg) {
+                                               double dt, int seg_rows, int nstrips, int strip,
+                                               int seg) {
   using T = typename P::T;
   const int lane = threadIdx.x & (kLanes - 1);
   const T dtdx = (T)(dt / kp.dx);
@@ -609,15 +674,23 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
   m.wmax = T(0.0);
 
   const int r0 = j_base - 2;
-#define E2D_ROW(ST, PF, k)                                                                 \
-  march_row<RS, P, ST, PF>(kp, in, out, m, r0 + (k), c0, flipx, dtdx, dtdy, ghost_col, out_col, \
-                           col, wlo, whi, want_rate)
-  E2D_ROW(0, true, 0);
-  E2D_ROW(0, true, 1);
-  E2D_ROW(kSlopes, true, 2);
-  E2D_ROW(kSlopes | kYFace | kXFace, true, 3);
-  for (int k = 4; k <= seg_rows + 2; ++k) E2D_ROW(kAll, true, k);
-  E2D_ROW(kSlopes | kYFace | kUpdate, false, seg_rows + 3);
+#define E2D_ROW(ST, PF, RT, k)                                                               \
+  march_row<RS, P, ST, PF, RT>(kp, in, out, m, r0 + (k), c0, flipx, dtdx, dtdy, ghost_col,   \
+                               out_col, col, wlo, whi)
+  E2D_ROW(0, true, false, 0);
+  E2D_ROW(0, true, false, 1);
+  E2D_ROW(kSlopes, true, false, 2);
+  E2D_ROW(kSlopes | kYFace | kXFace, true, false, 3);
+  // steady state, unrolled three times: the rotating row buffers of MarchState
+  // are then renamed instead of copied at every iteration
+  int k = 4;
+  for (; k + 2 <= seg_rows + 2; k += 3) {
+    E2D_ROW(kAll, true, RATE, k);
+    E2D_ROW(kAll, true, RATE, k + 1);
+    E2D_ROW(kAll, true, RATE, k + 2);
+  }
+  for (; k <= seg_rows + 2; ++k) E2D_ROW(kAll, true, RATE, k);
+  E2D_ROW(kSlopes | kYFace | kUpdate, false, RATE, seg_rows + 3);
 #undef E2D_ROW
   return m.wmax;
 }
@@ -658,18 +731,19 @@ __device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long lon
   return last;
 }
 
-template <int RS, class P>
+// LOOP = false: explicit dt (HydroRun::godunov_unsplit); true: run-loop mode
+template <int RS, class P, bool LOOP>
 __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in, Grid out,
                                                             double dt_arg, StepCtl *ctl,
                                                             int seg_rows, int nstrips, int nwaves) {
   using T = typename P::T;
-  if (ctl && !ctl->active) return;  // loop finished: nothing to do (uniform)
+  if (LOOP && !ctl->active) return;  // loop finished: nothing to do (uniform)
   const int wave = blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6);
   T wmax = T(0.0);
   if (wave < nwaves)
-    wmax = march<RS, P>(kp, in, out, ctl ? ctl->dt : dt_arg, ctl != nullptr, seg_rows, nstrips,
-                        wave % nstrips, wave / nstrips);
-  if (!ctl) return;
+    wmax = march<RS, P, LOOP>(kp, in, out, LOOP ? ctl->dt : dt_arg, seg_rows, nstrips,
+                              wave % nstrips, wave / nstrips);
+  if (!LOOP) return;
   // run-loop epilogue: next-step invDt (exact max, order independent) and the
   // last block advances the loop state
   if (reduce_rate_last_block(wmax, &ctl->invdt, &ctl->blocks_done) && threadIdx.x == 0) {
@@ -709,8 +783,8 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *ins
     if (ctl->active) {
       const int w = wave - bi.wave_begin;
       const bool even = (ctl->nstep & 1) == 0;
-      wmax = march<RS, P>(bi.kp, even ? bi.U : bi.U2, even ? bi.U2 : bi.U, ctl->dt, true,
-                          seg_rows, bi.nstrips, w % bi.nstrips, w / bi.nstrips);
+      wmax = march<RS, P, true>(bi.kp, even ? bi.U : bi.U2, even ? bi.U2 : bi.U, ctl->dt,
+                                seg_rows, bi.nstrips, w % bi.nstrips, w / bi.nstrips);
     }
   }
   const bool last = reduce_rate_last_block(wmax, inst >= 0 ? &ctls[inst].invdt : nullptr,
@@ -781,8 +855,12 @@ void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
   const int nwaves = nstrips * nsegs;
   const int waves_per_block = kBlock / kLanes;
   const int blocks = (nwaves + waves_per_block - 1) / waves_per_block;
-  hipLaunchKernelGGL((step_fused_kernel<RS, P>), dim3(blocks), dim3(kBlock), 0, st, kp, in, out,
-                     dt, ctl, seg_rows, nstrips, nwaves);
+  if (ctl)
+    hipLaunchKernelGGL((step_fused_kernel<RS, P, true>), dim3(blocks), dim3(kBlock), 0, st, kp, in,
+                       out, dt, ctl, seg_rows, nstrips, nwaves);
+  else
+    hipLaunchKernelGGL((step_fused_kernel<RS, P, false>), dim3(blocks), dim3(kBlock), 0, st, kp,
+                       in, out, dt, ctl, seg_rows, nstrips, nwaves);
 }
 
 template <class P>
@@ -799,7 +877,7 @@ void launch_fused_p(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
 }  // namespace
 
 int grid_planes(int precision) { return precision == E2D_PRECISION_FP32_SPLIT ? 8 : 4; }
-int grid_scalar_bytes(int precision) { return precision == E2D_PRECISION_FP32_SPLIT ? 4 : 8; }
+int grid_scalar_bytes(int precision) { return precision == E2D_PRECISION_FP64 ? 8 : 4; }
 
 KParams make_kparams(const e2d_params &p, int precision) {
   KParams k{};
@@ -847,6 +925,8 @@ KParams make_kparams(const e2d_params &p, int precision) {
   do {                                                    \
     if ((kp).prec == E2D_PRECISION_FP32_SPLIT)            \
       hipLaunchKernelGGL(KERNEL<PSplit>, __VA_ARGS__);    \
+    else if ((kp).prec == E2D_PRECISION_FP32)             \
+      hipLaunchKernelGGL(KERNEL<P32>, __VA_ARGS__);       \
     else                                                  \
       hipLaunchKernelGGL(KERNEL<P64>, __VA_ARGS__);       \
   } while (0)
@@ -917,6 +997,8 @@ hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, St
   if (seg_rows < 1) seg_rows = 1;
   if (kp.prec == E2D_PRECISION_FP32_SPLIT)
     launch_fused_p<PSplit>(kp, in, out, dt, ctl, seg_rows, st);
+  else if (kp.prec == E2D_PRECISION_FP32)
+    launch_fused_p<P32>(kp, in, out, dt, ctl, seg_rows, st);
   else
     launch_fused_p<P64>(kp, in, out, dt, ctl, seg_rows, st);
   return hipGetLastError();
@@ -939,6 +1021,10 @@ hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int
     if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, PSplit);
     else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, PSplit);
     else E2D_BATCH_LAUNCH(0, PSplit);
+  } else if (kp0.prec == E2D_PRECISION_FP32) {
+    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, P32);
+    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, P32);
+    else E2D_BATCH_LAUNCH(0, P32);
   } else {
     if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, P64);
     else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, P64);

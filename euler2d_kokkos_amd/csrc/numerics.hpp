@@ -113,6 +113,7 @@ __device__ __forceinline__ T limited_slope(T st, T q, T qplus, T qminus) {
 // slopes when slope_type == 0, limited slopes when 1 (minmod) or 2 (MC)
 template <class T>
 __device__ __forceinline__ T slope1(const SchemeT<T> &s, T q, T qplus, T qminus) {
+  // slope_type is uniform, so this is a scalar branch, not VALU work
   return s.slope_type == T(0.0) ? T(0.0) : limited_slope(s.slope_type, q, qplus, qminus);
 }
 

@@ -51,7 +51,8 @@ extern "C" {
 
 /* numeric precision of a run */
 #define E2D_PRECISION_FP64 0
-#define E2D_PRECISION_FP32_SPLIT 1
+#define E2D_PRECISION_FP32_SPLIT 1 /* hi+lo FP32 state, FP32 arithmetic */
+#define E2D_PRECISION_FP32 2       /* all-float comparator (reference USE_DOUBLE=OFF) */
 
 /* step schedules of the FP64 path */
 #define E2D_SCHEDULE_FUSED 0    /* one fused kernel per step (default) */

@@ -116,3 +116,29 @@ def test_split_tend_clamp(gpu_available):
     r32.run()
     assert r64.t == pytest.approx(0.05, rel=1e-15) and r32.t == pytest.approx(0.05, rel=1e-15)
     assert abs(r32.nStep - r64.nStep) <= 1
+
+
+def test_plain_fp32_comparator(gpu_available):
+    """The all-float comparator runs the same problem; the split state keeps the
+    conserved sums far better than plain FP32 does."""
+    from euler2d_kokkos_amd import E2D_PRECISION_FP32
+    p = O.default_params(nx=160, ny=120, problemType=1, cfl=0.7, nStepmax=600, tEnd=1e9,
+                         blast_radius=0.12, blast_center_x=0.45, blast_center_y=0.55,
+                         boundary_type_xmin=3, boundary_type_xmax=3, boundary_type_ymin=3,
+                         boundary_type_ymax=3)
+    out = {}
+    for name, prec in (("fp64", E2D_PRECISION_FP64), ("split", E2D_PRECISION_FP32_SPLIT),
+                       ("fp32", E2D_PRECISION_FP32)):
+        r = HydroRun(HydroParams(p.copy()), precision=prec)
+        s0 = r.conserved_sums(E2D_U)
+        r.run()
+        s1 = r.conserved_sums(r.current())
+        out[name] = (r.get_state(r.current()), np.abs(s1 - s0)[:2] / np.abs(s0[:2]))
+    ref = O.interior(out["fp64"][0])
+    l1_split = O.rel_l1(O.interior(out["split"][0]), ref)
+    l1_fp32 = O.rel_l1(O.interior(out["fp32"][0]), ref)
+    print(f"L1 split={l1_split.tolist()} fp32={l1_fp32.tolist()} "
+          f"drift split={out['split'][1].tolist()} fp32={out['fp32'][1].tolist()}")
+    assert l1_fp32[0] < 1e-2 and l1_split[0] <= L1_TOL
+    assert np.all(out["split"][1] <= 1e-12)
+    assert np.all(out["fp32"][1] > out["split"][1])
