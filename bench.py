@@ -6,13 +6,17 @@ HLLC, CFL 0.7), FP64, 2000 steps from t=0 -- one step is one timestep of the
 whole grid (dt reduction included, dt kept on the device).  The state lives in
 HBM before the timed region starts; nothing crosses PCIe inside it.
 
-  python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+  python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--workload blast]
 
-Prints one JSON line.  `value` is interior cell-updates per second over the
+The step kernel's tile (rows marched per wavefront) is autotuned on the initial
+state before the timed region unless --tile fixes it.  Prints one JSON line.  `value` is interior cell-updates per second over the
 timed region (host clock, stream synchronised on both sides).  `roofline` uses
 the step kernel's average duration from HIP events on the solver's stream and
 the algorithmic 64 bytes per cell-update (read 4 doubles, write 4 doubles).
-`cpu_baseline` times the CPU oracle (a restatement of the reference's
+`fp32_split` repeats the run in split precision (hi+lo FP32 state) and reports
+its distance from the FP64 result; `batched` times the seeded 512-instance
+ragged sweep (sizes 64..512 per side, FP32 split, 1000 steps each) advanced by
+one batched launch per step.  `cpu_baseline` times the CPU oracle (a restatement of the reference's
 implementation 0 with OpenMP) on this host over a bounded sample.
 """
 from __future__ import annotations
@@ -30,7 +34,15 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Mcell-updates/s on 1 MI355X: FP64 plain vs FP32/batched; % of HBM roofline"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 64.0
-DECK = ROOT / "euler2d_kokkos_amd" / "decks" / "blast_1024x1536.ini"
+DECKS = {
+    "blast": ("blast_1024x1536.ini", "blast 1024x1536 FP64 HLLC CFL 0.7 reflecting walls"),
+    "four_quadrant": ("four_quadrant_8192.ini", "four-quadrant 8192x8192 FP64 HLL CFL 0.8 "
+                                                "transmissive"),
+    "shocked_bubble": ("shocked_bubble_16384x8192.ini", "shocked bubble 16384x8192 FP64 HLLC "
+                                                        "CFL 0.8"),
+    "implode": ("implode_256.ini", "implode 256x256 FP64 HLLC CFL 0.8 reflecting walls"),
+}
+TILES = (8, 10, 12, 14, 16, 20, 24, 32, 48, 64)
 
 
 def parse_args():
@@ -43,12 +55,41 @@ def parse_args():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-split", action="store_true",
                     help="skip the FP32 split-precision comparison run")
-    ap.add_argument("--tile", type=int, default=14,
-                    help="rows marched by one wavefront of the step kernel")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="rows marched by one wavefront of the step kernel (0: autotune)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched 512-instance sweep")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--workload", default="blast",
+                    choices=["blast", "four_quadrant", "shocked_bubble", "implode"],
+                    help="deck to time (blast 1024x1536 is the headline workload)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="K",
                     help="only launch the plain whole-array copy kernel K times (the known-size "
                          "reference for rocprof FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
+
+
+def batch_line(steps: int) -> dict:
+    """The 512-instance ragged sweep (FP32 split, HLLC, 1000 steps each) in one
+    batched launch per step, replayed from hipGraphs; tile autotuned first."""
+    from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT
+    from euler2d_kokkos_amd.batch import HydroBatch, sweep_params
+
+    ps = sweep_params(steps=steps)
+    b = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
+    tile, _ = b.autotune(candidates=(16, 24, 32, 48, 64), steps=5)
+    b.init()
+    info = b.info()
+    t0 = time.perf_counter()
+    ms, active = b.run(steps, use_graph=True)
+    wall = time.perf_counter() - t0
+    done = sum(b.status(i)[1] for i in range(info["count"]))
+    cell_updates = sum(p.nx * p.ny * b.status(i)[1] for i, p in enumerate(ps))
+    b.close()
+    return {"value": round(cell_updates / wall / 1e6, 2), "unit": "Mcell-updates/s",
+            "instances": info["count"], "cells": info["cells"], "steps_each": steps,
+            "instance_steps_done": done, "still_active": active, "tile": tile,
+            "device_ms": round(ms, 3), "precision": "fp32_split", "launch": "hipGraph"}
 
 
 def cpu_baseline(deck: Path, seconds: float) -> dict:
@@ -71,12 +112,11 @@ def cpu_baseline(deck: Path, seconds: float) -> dict:
     value = p.nx * p.ny * steps / el / 1e6
     return {"value": round(value, 3), "unit": "Mcell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"blast 1024x1536 FP64, {steps} steps after 1 warm-up step "
-                      f"(of 2000), CPU oracle with OpenMP over {threads} threads, "
-                      f"{el:.1f} s"}
+            "sample": f"{deck.name}: {steps} steps after 1 warm-up step, CPU oracle "
+                      f"(FP64) with OpenMP over {threads} threads, {el:.1f} s"}
 
 
-def split_comparison(params, run64, steps, tile, cells, fp64_ms) -> dict:
+def split_comparison(params, run64, steps, cells, fp64_ms) -> dict:
     """Same workload in FP32 split precision: throughput, L1 error against the
     FP64 run's final state, and FP64-summed drift of mass and energy."""
     import numpy as np
@@ -84,7 +124,7 @@ def split_comparison(params, run64, steps, tile, cells, fp64_ms) -> dict:
     from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_U, HydroRun
 
     run = HydroRun(params, precision=E2D_PRECISION_FP32_SPLIT)
-    run.set_tile(tile)
+    run.autotune(candidates=TILES, steps=10)
     run.run(20)
     run.synchronize()
     run.init_problem()
@@ -107,7 +147,7 @@ def split_comparison(params, run64, steps, tile, cells, fp64_ms) -> dict:
             "rel_l1_vs_fp64": [float(x) for x in l1],
             "mass_drift": float(abs(s1[0] - s0[0]) / abs(s0[0])),
             "energy_drift": float(abs(s1[1] - s0[1]) / abs(s0[1])),
-            "final_t": run.t}
+            "tile": run.get_tile(), "final_t": run.t}
 
 
 def main() -> int:
@@ -122,10 +162,14 @@ def main() -> int:
                                     E2D_SCHEDULE_FUSED, HydroParams, HydroRun, device_info)
 
     dev, ncu = device_info()
-    params = HydroParams.from_ini(DECK)
+    deck = ROOT / "euler2d_kokkos_amd" / "decks" / DECKS[args.workload][0]
+    params = HydroParams.from_ini(deck)
     params.nStepmax = max(params.nStepmax, args.steps)
     run = HydroRun(params, schedule=E2D_SCHEDULE_FUSED)
-    run.set_tile(args.tile)
+    if args.tile > 0:
+        run.set_tile(args.tile)
+    elif args.calibrate_copy <= 0:
+        run.autotune(candidates=TILES, steps=10)
 
     if args.calibrate_copy > 0:
         from euler2d_kokkos_amd import E2D_U, E2D_U2
@@ -168,10 +212,10 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (blast initial condition generated on the GPU)",
-        "config": {"workload": "blast 1024x1536 FP64 HLLC CFL 0.7 reflecting walls, "
-                               "2000 steps from t=0", "nx": params.nx, "ny": params.ny,
-                   "precision": "fp64", "schedule": "fused", "device": dev, "cus": ncu},
+        "data": f"synthetic ({args.workload} initial condition generated on the GPU)",
+        "config": {"workload": f"{DECKS[args.workload][1]}, {args.steps} steps from t=0",
+                   "nx": params.nx, "ny": params.ny, "precision": "fp64", "schedule": "fused",
+                   "tile": run.get_tile(), "device": dev, "cus": ncu},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None},
@@ -179,10 +223,11 @@ def main() -> int:
         "final_t": run.t,
     }
     if not args.no_split:
-        line["fp32_split"] = split_comparison(params, run, args.steps, args.tile, cells,
-                                              per_step_ms)
+        line["fp32_split"] = split_comparison(params, run, args.steps, cells, per_step_ms)
+    if not args.no_batch:
+        line["batched"] = batch_line(args.batch_steps)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(DECK, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(deck, args.cpu_seconds)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

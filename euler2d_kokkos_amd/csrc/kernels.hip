@@ -10,6 +10,8 @@
 // instantiation computes in FP32 on a hi+lo FP32 conserved state.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "kernels.hpp"
 
 #pragma clang fp contract(off)
@@ -19,6 +21,44 @@ namespace e2d {
 namespace {
 
 // ----------------------------------------------------- storage policies
+// Element (plane, row, column) of a grid addressed as a wave-uniform row base
+// plus a 32-bit per-lane byte offset `cb`: the base stays in scalar registers
+// and the access uses the scalar-base + vector-offset form of global memory
+// instructions, with no per-lane 64-bit address arithmetic.  `row` (and the
+// grid) must be the same in every lane of the wavefront: the base is taken
+// from the first active lane.
+template <class S>
+using global_ptr = __attribute__((address_space(1))) S *;
+
+// byte address of the start of `row` in plane 0 (wave-uniform)
+template <class S>
+__device__ __forceinline__ unsigned long long row_addr(const Grid &g, int row) {
+  return reinterpret_cast<unsigned long long>(g.p) +
+         (unsigned long long)((long long)row * g.pitch * (long long)sizeof(S));
+}
+
+template <class S>
+__device__ __forceinline__ global_ptr<S> plane_elem(unsigned long long ra, const Grid &g,
+                                                    int plane, unsigned cb) {
+  const unsigned long long b =
+      ra + (unsigned long long)plane * (unsigned long long)(g.plane * (long long)sizeof(S));
+  // readfirstlane keeps the compiler from folding the lane offset into a
+  // per-lane 64-bit base (on a value already uniform it emits nothing)
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  global_ptr<char> base = (global_ptr<char>)(((unsigned long long)hi << 32) | lo);
+  return (global_ptr<S>)(base + cb);
+}
+
+// A loop-invariant lane offset passed through an empty asm once per march
+// row: its zero extension then stays next to the row's accesses instead of
+// being hoisted out of the loop as a 64-bit value (which loses the
+// scalar-base form).  Costs one register move per row.
+__device__ __forceinline__ unsigned pin_offset(unsigned cb) {
+  asm volatile("" : "+v"(cb));
+  return cb;
+}
+
 // FP64: four double planes; the state is the value itself.
 struct P64 {
   using T = double;
@@ -35,12 +75,24 @@ struct P64 {
     double *p = static_cast<double *>(g.p);
     for (int v = 0; v < 4; v++) p[v * g.plane + off] = c.u[v];
   }
+  static __device__ __forceinline__ Cell load_rc(const Grid &g, int row, unsigned cb) {
+    const unsigned long long ra = row_addr<double>(g, row);
+    Cell c;
+    for (int v = 0; v < 4; v++) c.u[v] = *plane_elem<double>(ra, g, v, cb);
+    return c;
+  }
+  static __device__ __forceinline__ void store_rc(const Grid &g, int row, unsigned cb,
+                                                  const Cell &c) {
+    const unsigned long long ra = row_addr<double>(g, row);
+    for (int v = 0; v < 4; v++) *plane_elem<double>(ra, g, v, cb) = c.u[v];
+  }
   static __device__ __forceinline__ void negate(Cell &c, int v) { c.u[v] = -c.u[v]; }
   static __device__ __forceinline__ Cell from_double(double d, double e, double mx, double my) {
     return Cell{{d, e, mx, my}};
   }
   static __device__ __forceinline__ double value(const Cell &c, int v) { return c.u[v]; }
   static __device__ __forceinline__ const SchemeT<T> &scheme(const KParams &kp) { return kp.s; }
+  static __device__ __forceinline__ const RateT<T> &rate(const KParams &kp) { return kp.rd; }
   static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
     prim_no_c(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], q);
   }
@@ -94,6 +146,23 @@ struct PSplit {
       p[(2 * v + 1) * g.plane + off] = c.lo[v];
     }
   }
+  static __device__ __forceinline__ Cell load_rc(const Grid &g, int row, unsigned cb) {
+    const unsigned long long ra = row_addr<float>(g, row);
+    Cell c;
+    for (int v = 0; v < 4; v++) {
+      c.hi[v] = *plane_elem<float>(ra, g, 2 * v, cb);
+      c.lo[v] = *plane_elem<float>(ra, g, 2 * v + 1, cb);
+    }
+    return c;
+  }
+  static __device__ __forceinline__ void store_rc(const Grid &g, int row, unsigned cb,
+                                                  const Cell &c) {
+    const unsigned long long ra = row_addr<float>(g, row);
+    for (int v = 0; v < 4; v++) {
+      *plane_elem<float>(ra, g, 2 * v, cb) = c.hi[v];
+      *plane_elem<float>(ra, g, 2 * v + 1, cb) = c.lo[v];
+    }
+  }
   static __device__ __forceinline__ void negate(Cell &c, int v) {
     c.hi[v] = -c.hi[v];
     c.lo[v] = -c.lo[v];
@@ -111,6 +180,7 @@ struct PSplit {
     return (double)c.hi[v] + (double)c.lo[v];
   }
   static __device__ __forceinline__ const SchemeT<T> &scheme(const KParams &kp) { return kp.sf; }
+  static __device__ __forceinline__ const RateT<T> &rate(const KParams &kp) { return kp.rf; }
   // FP32 conversion; cells whose internal energy is a small fraction of the
   // total energy (kinetic-dominated: p = (gamma-1)(E - rho v^2/2) cancels)
   // are converted again in FP64 from hi + lo.
@@ -162,12 +232,24 @@ struct P32 {
     float *p = static_cast<float *>(g.p);
     for (int v = 0; v < 4; v++) p[v * g.plane + off] = c.u[v];
   }
+  static __device__ __forceinline__ Cell load_rc(const Grid &g, int row, unsigned cb) {
+    const unsigned long long ra = row_addr<float>(g, row);
+    Cell c;
+    for (int v = 0; v < 4; v++) c.u[v] = *plane_elem<float>(ra, g, v, cb);
+    return c;
+  }
+  static __device__ __forceinline__ void store_rc(const Grid &g, int row, unsigned cb,
+                                                  const Cell &c) {
+    const unsigned long long ra = row_addr<float>(g, row);
+    for (int v = 0; v < 4; v++) *plane_elem<float>(ra, g, v, cb) = c.u[v];
+  }
   static __device__ __forceinline__ void negate(Cell &c, int v) { c.u[v] = -c.u[v]; }
   static __device__ __forceinline__ Cell from_double(double d, double e, double mx, double my) {
     return Cell{{(float)d, (float)e, (float)mx, (float)my}};
   }
   static __device__ __forceinline__ double value(const Cell &c, int v) { return (double)c.u[v]; }
   static __device__ __forceinline__ const SchemeT<T> &scheme(const KParams &kp) { return kp.sf; }
+  static __device__ __forceinline__ const RateT<T> &rate(const KParams &kp) { return kp.rf; }
   static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
     prim_no_c(kp.sf, c.u[0], c.u[1], c.u[2], c.u[3], q);
   }
@@ -217,6 +299,16 @@ __device__ __forceinline__ int ghost_map(int i, int n, int bclo, int bchi, bool 
     return i - n;
   }
   return i;
+}
+
+// The same map for a wave-uniform row index, written as selects so that it
+// compiles to scalar selects rather than a tree of scalar branches.
+__device__ __forceinline__ int ghost_map_uniform(int i, int n, int bclo, int bchi, bool &flip) {
+  const int lo = bclo == 1 ? 3 - i : (bclo == 2 ? 2 : n + i);
+  const int hi = bchi == 1 ? 2 * n + 3 - i : (bchi == 2 ? n + 1 : i - n);
+  const bool below = i < 2, above = i > n + 1;
+  flip = (below && bclo == 1) || (above && bchi == 1);
+  return below ? lo : (above ? hi : i);
 }
 
 __device__ __forceinline__ unsigned long long dbits(double x) {
@@ -365,7 +457,6 @@ template <class P>
 __global__ void dt_reduce_kernel(KParams kp, Grid U, unsigned long long *invdt_bits) {
   using T = typename P::T;
   const long long n = (long long)kp.nx * kp.ny;
-  const T dx = (T)kp.dx, dy = (T)kp.dy;
   T m = T(0.0);
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
        idx += (long long)gridDim.x * blockDim.x) {
@@ -373,7 +464,7 @@ __global__ void dt_reduce_kernel(KParams kp, Grid U, unsigned long long *invdt_b
     const int i = 2 + (int)(idx - (long long)(j - 2) * kp.nx);
     Q4T<T> q;
     P::prim(kp, P::load(U, (long long)j * U.pitch + i), q);
-    m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, dx, dy));
+    m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, P::rate(kp)));
   }
   m = wave_max(m);
   const double md = (double)m;
@@ -485,22 +576,24 @@ __global__ void update_kernel(KParams kp, Grid U, Grid Fx, Grid Fy) {
 
 // Whole-wavefront lane shifts by one with DPP (GFX9 wave_shr:1 / wave_shl:1):
 // a full-rate VALU move instead of an LDS-routed permute.  Lane 0 (shift up)
-// and lane 63 (shift down) keep their own value; the march never uses them.
+// and lane 63 (shift down) read zero (bound_ctrl); the march never uses the
+// results of those lanes: the x-face of lane 1, the slopes of lanes 0 and 63
+// and the x-face of lane 63 feed only halo lanes.
 constexpr int kDppWaveShr1 = 0x138;  // lane i reads lane i-1
 constexpr int kDppWaveShl1 = 0x130;  // lane i reads lane i+1
 
 template <int CTRL>
 __device__ __forceinline__ float dpp_shift(float x) {
   const int v = __float_as_int(x);
-  return __int_as_float(__builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true));
 }
 
 template <int CTRL>
 __device__ __forceinline__ double dpp_shift(double x) {
   const long long b = __double_as_longlong(x);
   const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
-  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
-  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+  const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
   return __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
 }
 
@@ -551,8 +644,8 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   if constexpr (PREFETCH) {
     bool flipy;
     const int rc = min(r + 1, kp.ny + 3);
-    const int r0 = ghost_map(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
-    m.un = P::load(in, (long long)r0 * in.pitch + c0);
+    const int r0 = ghost_map_uniform(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
+    m.un = P::load_rc(in, r0, pin_offset((unsigned)c0 * sizeof(T)));
     if (flipx) P::negate(m.un, IU);
     if (flipy) P::negate(m.un, IV);
   }
@@ -564,10 +657,12 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   if constexpr ((STAGES & kSlopes) != 0) {
     const Q4T<T> qpx = shfl_dn1(m.qm1);
     const Q4T<T> qmx = shfl_up1(m.qm1);
-    Q4T<T> dqX, dqY;
-    for (int v = 0; v < 4; v++) {
-      dqX.v[v] = slope1(s, m.qm1.v[v], qpx.v[v], qmx.v[v]);
-      dqY.v[v] = slope1(s, m.qm1.v[v], q0.v[v], m.qm2.v[v]);
+    Q4T<T> dqX = {}, dqY = {};
+    if (s.slope_type != T(0.0)) {  // uniform: one scalar branch per row
+      for (int v = 0; v < 4; v++) {
+        dqX.v[v] = limited_slope(s.slope_type, m.qm1.v[v], qpx.v[v], qmx.v[v]);
+        dqY.v[v] = limited_slope(s.slope_type, m.qm1.v[v], q0.v[v], m.qm2.v[v]);
+      }
     }
     const SourcesT<T> src = trace_sources(s, m.qm1, dqX, dqY);
     Q4T<T> fymin;
@@ -597,18 +692,18 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   // ghost cells of row r owned by this wave: materialise them in both arrays
   const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
   if ((ghost_col || row_ghost) && r >= row_lo && r <= row_hi) {
-    const long long off = (long long)r * in.pitch + col;
-    P::store(in, off, uc);
-    P::store(out, off, uc);
+    const unsigned cb = pin_offset((unsigned)col * sizeof(T));
+    P::store_rc(in, r, cb, uc);
+    P::store_rc(out, r, cb, uc);
   }
   if constexpr ((STAGES & kUpdate) != 0) {
     const int b = r - 2;
     if (out_col && b <= kp.ny + 1) {
-      P::store(out, (long long)b * out.pitch + col, unew);
+      P::store_rc(out, b, pin_offset((unsigned)col * sizeof(T)), unew);
       if constexpr (RATE) {  // next-step CFL rate of the new state
         Q4T<T> qn;
         P::prim(kp, unew, qn);
-        m.wmax = Ops<T>::mx(m.wmax, cfl_rate_q(s, qn, (T)kp.dx, (T)kp.dy));
+        m.wmax = Ops<T>::mx(m.wmax, cfl_rate_q(s, qn, P::rate(kp)));
       }
     }
   }
@@ -663,8 +758,9 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
   MarchState<P> m;
   {
     bool flipy;
-    const int r0 = ghost_map(min(j_base - 2, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
-    m.un = P::load(in, (long long)r0 * in.pitch + c0);
+    const int r0 =
+        ghost_map_uniform(min(j_base - 2, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
+    m.un = P::load_rc(in, r0, (unsigned)c0 * sizeof(T));
     if (flipx) P::negate(m.un, IU);
     if (flipy) P::negate(m.un, IV);
   }
@@ -738,7 +834,10 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
                                                             int seg_rows, int nstrips, int nwaves) {
   using T = typename P::T;
   if (LOOP && !ctl->active) return;  // loop finished: nothing to do (uniform)
-  const int wave = blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6);
+  // the wave index is uniform: every row index and the row ghost map derived
+  // from it are then scalar values and scalar control flow
+  const int wave =
+      __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6));
   T wmax = T(0.0);
   if (wave < nwaves)
     wmax = march<RS, P, LOOP>(kp, in, out, LOOP ? ctl->dt : dt_arg, seg_rows, nstrips,
@@ -760,12 +859,14 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
 // in/out arrays follow the instance's own step parity.  The last block of the
 // grid advances every instance's loop state.
 template <int RS, class P>
-__global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *insts, int ninst,
+__global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__restrict__ insts,
+                                                            int ninst,
                                                             StepCtl *ctls, int nwaves,
                                                             int seg_rows,
                                                             unsigned int *blocks_done) {
   using T = typename P::T;
-  const int wave = blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6);
+  const int wave =
+      __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6));
   T wmax = T(0.0);
   int inst = -1;
   if (wave < nwaves) {
@@ -783,8 +884,13 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *ins
     if (ctl->active) {
       const int w = wave - bi.wave_begin;
       const bool even = (ctl->nstep & 1) == 0;
-      wmax = march<RS, P, true>(bi.kp, even ? bi.U : bi.U2, even ? bi.U2 : bi.U, ctl->dt,
-                                seg_rows, bi.nstrips, w % bi.nstrips, w / bi.nstrips);
+      // by-value copies: the march then reads its parameters from scalar
+      // registers instead of reloading them around every store
+      const KParams kp = bi.kp;
+      const Grid gin = even ? bi.U : bi.U2;
+      const Grid gout = even ? bi.U2 : bi.U;
+      wmax = march<RS, P, true>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips, w % bi.nstrips,
+                                w / bi.nstrips);
     }
   }
   const bool last = reduce_rate_last_block(wmax, inst >= 0 ? &ctls[inst].invdt : nullptr,
@@ -918,6 +1024,14 @@ KParams make_kparams(const e2d_params &p, int precision) {
   k.cfl = p.cfl;
   k.tEnd = p.tEnd;
   k.kin_frac = 0.1f;
+  // exact powers of two (normal, with a normal reciprocal) in both FP64 and FP32
+  auto pow2 = [](double h) {
+    int e = 0;
+    return h > 0.0 && std::frexp(h, &e) == 0.5 && e > -125 && e < 126;
+  };
+  k.rd = RateT<double>{p.dx, p.dy, 1.0 / p.dx, 1.0 / p.dy, pow2(p.dx), pow2(p.dy)};
+  k.rf = RateT<float>{k.dxf, k.dyf, (float)(1.0 / p.dx), (float)(1.0 / p.dy), pow2(p.dx),
+                      pow2(p.dy)};
   return k;
 }
 

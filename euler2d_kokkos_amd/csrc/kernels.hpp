@@ -31,6 +31,8 @@ struct KParams {
   int nStepmax;
   double dx, dy, cfl, tEnd;
   float dxf, dyf;
+  RateT<double> rd;     // cell sizes of the CFL rate (FP64 / FP32 arithmetic)
+  RateT<float> rf;
   // split path: primitive conversion redone in FP64 where the internal energy
   // is below this fraction of the total energy (p < kin_frac * (gamma-1) * E)
   float kin_frac;

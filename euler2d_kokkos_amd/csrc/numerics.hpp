@@ -89,6 +89,34 @@ __device__ __forceinline__ T cfl_rate_q(const SchemeT<T> &s, const Q4T<T> &q, T 
   return O::div(vx, dx) + O::div(vy, dy);
 }
 
+// Cell sizes for the CFL rate.  When a cell size is an exact power of two its
+// reciprocal is exact and x / dx == x * (1/dx) for every x (IEEE division by
+// 2^-k is an exact scaling, overflow and NaN included), so the division is
+// replaced by a multiplication; other sizes keep the division.
+template <class T>
+struct RateT {
+  T dx, dy, idx, idy;
+  int pow2x, pow2y;
+};
+
+template <class T>
+__device__ __forceinline__ T cfl_rate_q(const SchemeT<T> &s, const Q4T<T> &q, const RateT<T> &r) {
+  using O = Ops<T>;
+  const T c = O::sqroot(O::div(s.gamma0 * (q.v[IP]), q.v[ID]));
+  const T vx = c + O::ab(q.v[IU]);
+  const T vy = c + O::ab(q.v[IV]);
+  T rx, ry;
+  if (r.pow2x)  // uniform
+    rx = vx * r.idx;
+  else
+    rx = O::div(vx, r.dx);
+  if (r.pow2y)
+    ry = vy * r.idy;
+  else
+    ry = O::div(vy, r.dy);
+  return rx + ry;
+}
+
 __device__ __forceinline__ double cfl_rate(const Scheme &s, double u0, double u1, double u2,
                                            double u3, double dx, double dy) {
   Q4 q;
