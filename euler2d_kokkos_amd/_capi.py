@@ -111,6 +111,12 @@ _PROTOTYPES = {
     "e2d_copy_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "e2d_conserved_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "e2d_timers": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "e2d_save_vtk": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    "e2d_write_vti": (C.c_int, [C.POINTER(E2DParams), C.c_void_p, C.c_int, C.c_char_p,
+                                C.c_char_p, C.c_int]),
+    "e2d_radial_profile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "e2d_write_npy": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int64]),
     "e2d_set_schedule": (C.c_int, [C.c_void_p, C.c_int]),
     "e2d_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
     "e2d_get_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

@@ -40,7 +40,7 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["config.cpp", "params.cpp", "api.cpp", "kernels.hip"]
+SOURCES = ["config.cpp", "params.cpp", "api.cpp", "io.cpp", "kernels.hip"]
 
 
 def hipcc() -> str:

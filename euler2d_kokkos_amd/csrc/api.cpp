@@ -12,10 +12,13 @@
 #include <string>
 #include <vector>
 
+#include <cmath>
+
 #include "config.hpp"
 #include "euler2d_hip.h"
 #include "kernels.hpp"
 #include "params.hpp"
+#include "status.hpp"
 
 using e2d::Grid;
 
@@ -42,6 +45,8 @@ int hip_fail(hipError_t e, const char *what) {
 constexpr int kChunk = 256;
 
 }  // namespace
+
+int e2d::report(int code, const std::string &msg) { return fail(code, msg); }
 
 struct e2d_config {
   e2d::ConfigMap map;
@@ -694,6 +699,65 @@ int e2d_conserved_sums(e2d_run *r, int which, double sums[4]) {
   HIP_TRY(hipMemcpyAsync(sums, r->sums4, 4 * sizeof(double), hipMemcpyDeviceToHost, r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
   return E2D_OK;
+}
+
+int e2d_save_vtk(e2d_run *r, int which, const char *output_dir, const char *output_prefix,
+                 int step) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (which != E2D_U && which != E2D_U2) return fail(E2D_ERR_ARG, "which must be E2D_U or E2D_U2");
+  return guarded([&]() -> int {
+    std::vector<double> host((size_t)4 * r->p.nx * r->p.ny);
+    int rc2 = e2d_copy_state(r, which, host.data(), E2D_LAYOUT_INTERIOR);
+    if (rc2) return rc2;
+    return e2d_write_vti(&r->p, host.data(), r->precision == E2D_PRECISION_FP32 ? 1 : 0,
+                         output_dir, output_prefix, step);
+  });
+}
+
+int e2d_radial_profile(e2d_run *r, int which, int nbins, double *distances, double *density,
+                       int64_t *counts) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (which != E2D_U && which != E2D_U2) return fail(E2D_ERR_ARG, "which must be E2D_U or E2D_U2");
+  if (nbins < 1 || !distances || !density) return fail(E2D_ERR_ARG, "bad argument");
+  const e2d_params &p = r->p;
+  // ComputeRadialProfileFunctor::apply, ComputeRadialProfileFunctor.h:83-128
+  e2d::RadialGeom geo{};
+  geo.xmin = p.xmin;
+  geo.ymin = p.ymin;
+  geo.cx = (p.xmin + p.xmax) / 2;
+  geo.cy = (p.ymin + p.ymax) / 2;
+  const double Dx = (p.xmax - p.xmin) / 2;
+  const double Dy = (p.ymax - p.ymin) / 2;
+  geo.max_radial_distance = std::sqrt(Dx * Dx + Dy * Dy);
+  geo.nbins = nbins;
+  return guarded([&]() -> int {
+    void *dev = nullptr;
+    const size_t bytes = (size_t)nbins * (sizeof(unsigned long long) + sizeof(double));
+    HIP_TRY(hipMalloc(&dev, bytes));
+    auto *cnt = static_cast<unsigned long long *>(dev);
+    auto *sum = reinterpret_cast<double *>(cnt + nbins);
+    std::vector<unsigned long long> hc(nbins);
+    std::vector<double> hs(nbins);
+    hipError_t e = e2d::launch_radial_profile(r->kp, state(r, which), geo, cnt, sum, r->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hc.data(), cnt, sizeof(unsigned long long) * nbins,
+                         hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hs.data(), sum, sizeof(double) * nbins, hipMemcpyDeviceToHost,
+                         r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return hip_fail(e, "radial profile");
+    const double dr = geo.max_radial_distance / nbins;
+    for (int i = 0; i < nbins; ++i) {
+      distances[i] = (i + 0.5) * dr;
+      density[i] = hs[i] / (int)hc[i];  // empty bin: 0/0, as the reference
+      if (counts) counts[i] = (int64_t)hc[i];
+    }
+    return E2D_OK;
+  });
 }
 
 int e2d_timers(e2d_run *r, double out[5]) {

@@ -902,6 +902,55 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__r
   }
 }
 
+// ----------------------------------------------------------- radial profile
+// ComputeRadialProfileFunctor (ComputeRadialProfileFunctor.h:140-165): every
+// cell of the array, ghosts included, adds its density to the bin of its
+// distance from the box centre.  Counts are exact; the density sums are FP64
+// atomics, so their last bits depend on the order, as in the reference.  Cells
+// whose bin is >= nbins (ghost cells past the half-diagonal) are left out: the
+// reference increments outside its histogram there.
+constexpr int kProfileLdsBins = 1024;
+
+template <class P>
+__global__ void radial_profile_kernel(KParams kp, Grid U, RadialGeom geo,
+                                      unsigned long long *counts, double *sums) {
+  __shared__ double lsum[kProfileLdsBins];
+  __shared__ unsigned int lcnt[kProfileLdsBins];
+  const bool use_lds = geo.nbins <= kProfileLdsBins;
+  if (use_lds)
+    for (int b = threadIdx.x; b < geo.nbins; b += blockDim.x) {
+      lsum[b] = 0.0;
+      lcnt[b] = 0u;
+    }
+  __syncthreads();
+  const long long n = (long long)kp.isize * kp.jsize;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / kp.isize);
+    const int i = (int)(idx - (long long)j * kp.isize);
+    const double x = geo.xmin + kp.dx / 2 + (i - 2) * kp.dx;
+    const double y = geo.ymin + kp.dy / 2 + (j - 2) * kp.dy;
+    const double distance = sqrt((x - geo.cx) * (x - geo.cx) + (y - geo.cy) * (y - geo.cy));
+    const int bin = (int)(distance / geo.max_radial_distance * geo.nbins);
+    if (bin < 0 || bin >= geo.nbins) continue;
+    const double rho = P::value(P::load(U, (long long)j * U.pitch + i), ID);
+    if (use_lds) {
+      atomicAdd(&lsum[bin], rho);
+      atomicAdd(&lcnt[bin], 1u);
+    } else {
+      atomicAdd(&sums[bin], rho);
+      atomicAdd(&counts[bin], 1ull);
+    }
+  }
+  if (!use_lds) return;
+  __syncthreads();
+  for (int b = threadIdx.x; b < geo.nbins; b += blockDim.x)
+    if (lcnt[b]) {
+      atomicAdd(&sums[b], lsum[b]);
+      atomicAdd(&counts[b], (unsigned long long)lcnt[b]);
+    }
+}
+
 // -------------------------------------------------- conserved-variable sums
 // Deterministic two-level FP64 sum: block b sums rows b, b+G, ... in a fixed tree.
 constexpr int kSumBlocks = 512;
@@ -1153,6 +1202,21 @@ int sums_partials_needed(const KParams &) { return kSumBlocks * 4; }
 hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st) {
   E2D_DISPATCH_P(kp, sums_partial_kernel, dim3(kSumBlocks), dim3(256), 0, st, kp, U, partials);
   hipLaunchKernelGGL(sums_final_kernel, dim3(1), dim3(256), 0, st, partials, kSumBlocks, out4);
+  return hipGetLastError();
+}
+
+hipError_t launch_radial_profile(const KParams &kp, Grid U, const RadialGeom &geo,
+                                 unsigned long long *counts, double *sums, hipStream_t st) {
+  if (geo.nbins < 1) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetAsync(counts, 0, sizeof(unsigned long long) * geo.nbins, st);
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(sums, 0, sizeof(double) * geo.nbins, st);
+  if (e != hipSuccess) return e;
+  const long long n = (long long)kp.isize * kp.jsize;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  E2D_DISPATCH_P(kp, radial_profile_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U, geo,
+                 counts, sums);
   return hipGetLastError();
 }
 

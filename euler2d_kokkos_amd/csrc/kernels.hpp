@@ -59,6 +59,12 @@ struct BatchInst {
   int pad[2];
 };
 
+// Geometry of the radial profile (ComputeRadialProfileFunctor.h:83-100)
+struct RadialGeom {
+  double xmin, ymin, cx, cy, max_radial_distance;
+  int nbins;
+};
+
 // fused step geometry: one wave of 64 lanes covers 60 output columns (plus a
 // 2-column halo on each side) and marches over `seg_rows` output rows
 constexpr int kLanes = 64;
@@ -94,6 +100,9 @@ int batch_waves(const KParams &kp, int seg_rows, int *nstrips);
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
                              StepCtl *ctls_dev, int nwaves, int seg_rows,
                              unsigned int *blocks_done, hipStream_t st);
+// per-bin cell counts and density sums of the radial profile (nbins entries each)
+hipError_t launch_radial_profile(const KParams &kp, Grid U, const RadialGeom &geo,
+                                 unsigned long long *counts, double *sums, hipStream_t st);
 // whole-array copy (all planes, ghosts and padding included)
 hipError_t launch_copy(const KParams &kp, Grid src, Grid dst, hipStream_t st);
 

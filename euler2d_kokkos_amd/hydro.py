@@ -250,6 +250,42 @@ class HydroRun:
         check(load_library().e2d_timers(self._h, out))
         return dict(zip(("boundaries", "godunov", "primitives", "fluxes", "update"), out[:]))
 
+    def save_vtk(self, which: int, output_dir: str | Path = ".", output_prefix: str = "output",
+                 step: int | None = None) -> Path:
+        """HydroRun::saveVTK: writes <output_dir>/<output_prefix>_<step:07d>.vti."""
+        step = self.nStep if step is None else step
+        check(load_library().e2d_save_vtk(self._h, which, str(output_dir).encode(),
+                                          output_prefix.encode(), step))
+        return Path(output_dir) / f"{output_prefix}_{step:07d}.vti"
+
+    def radial_profile(self, nbins: int | None = None, which: int = _capi.E2D_U):
+        """ComputeRadialProfileFunctor: (bin centres, mean density, cell counts)."""
+        n = self.params.blast_nbins if nbins is None else nbins
+        dist = np.empty(n)
+        dens = np.empty(n)
+        cnt = np.empty(n, dtype=np.int64)
+        check(load_library().e2d_radial_profile(self._h, which, n, dist.ctypes.data,
+                                                dens.ctypes.data, cnt.ctypes.data))
+        return dist, dens, cnt
+
+
+def write_vti(params, interior: np.ndarray, output_dir: str | Path = ".",
+              output_prefix: str = "output", step: int = 0, float32: bool = False) -> Path:
+    """The .vti file of HydroRun::saveVTK from a host array [var][j][i] (nx*ny)."""
+    p = params.raw if isinstance(params, HydroParams) else params
+    a = np.ascontiguousarray(interior, dtype=np.float64)
+    if a.shape != (4, p.ny, p.nx):
+        raise ValueError(f"expected shape (4, {p.ny}, {p.nx}), got {a.shape}")
+    check(load_library().e2d_write_vti(C.byref(p), a.ctypes.data, int(float32),
+                                       str(output_dir).encode(), output_prefix.encode(), step))
+    return Path(output_dir) / f"{output_prefix}_{step:07d}.vti"
+
+
+def write_npy(path: str | Path, data: np.ndarray) -> None:
+    """cnpy::npy_save of a 1-D double array."""
+    a = np.ascontiguousarray(data, dtype=np.float64).ravel()
+    check(load_library().e2d_write_npy(str(path).encode(), a.ctypes.data, a.size))
+
 
 def device_info() -> tuple[str, int]:
     buf = C.create_string_buffer(256)

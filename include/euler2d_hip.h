@@ -200,6 +200,29 @@ int e2d_synchronize(e2d_run *run);
 int e2d_time_loop(e2d_run *run, int nsteps, double *elapsed_ms, int *executed, double *t,
                   int *nStep);
 
+/* ------------------------------------------------------------- output */
+/* HydroRun::saveVTK (src/HydroRun.h:502-609): ASCII VTK ImageData file
+ * <output_dir>/<output_prefix>_<7-digit step>.vti with the interior cells of
+ * U / U2, i fastest, arrays rho, E, mx, my (Float32 for E2D_PRECISION_FP32,
+ * else Float64) */
+int e2d_save_vtk(e2d_run *run, int which, const char *output_dir, const char *output_prefix,
+                 int step);
+/* the same file from a host array in E2D_LAYOUT_INTERIOR order ([var][j][i],
+ * nx*ny per variable) */
+int e2d_write_vti(const e2d_params *p, const double *interior, int use_float32,
+                  const char *output_dir, const char *output_prefix, int step);
+/* ComputeRadialProfileFunctor::apply (src/ComputeRadialProfileFunctor.h:83-128)
+ * without the file output: bin centres (i+0.5)*dr, dr = half-diagonal/nbins,
+ * and the mean density of every cell (ghosts included) whose distance from the
+ * box centre falls in the bin (NaN for an empty bin); counts may be NULL.
+ * Cells past the last bin are left out (the reference writes outside its
+ * histogram for them). */
+int e2d_radial_profile(e2d_run *run, int which, int nbins, double *distances, double *density,
+                       int64_t *counts);
+/* cnpy::npy_save of a 1-D double array (src/cnpy/cnpy.h:129-187,343-376):
+ * numpy format 1.0, header padded to a multiple of 16 bytes */
+int e2d_write_npy(const char *path, const double *data, int64_t n);
+
 /* ------------------------------------------------------------------ batch */
 /* Many independent problems advanced together (no reference counterpart: the
  * reference runs one grid per process, src/main.cpp:86).  Every instance keeps

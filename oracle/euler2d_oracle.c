@@ -37,6 +37,13 @@
  *                               solution, dt closed form [test_oracle_step.py]
  *   Init* functors           -> closed-form states and region tests
  *                               [test_oracle_step.py]
+ *   saveVTK                  -> XML structure parsed independently, values
+ *                               read back at 6 significant digits, file name
+ *                               rule [test_output_formats.py]
+ *   npy writer               -> numpy's own loader [test_output_formats.py]
+ *   radial profile           -> brute-force numpy binning and a uniform field
+ *                               whose profile is the field value
+ *                               [test_output_formats.py]
  */
 #include "euler2d_oracle.h"
 
@@ -728,6 +735,102 @@ int ora_set_array(ora_run *r, int which, const double *in) {
   if (which < 0 || which > 4) return -1;
   memcpy(r->arr[which], in, (size_t)r->p.isize * r->p.jsize * 4 * sizeof(double));
   return 0;
+}
+
+/* HydroRun::saveVTK, HydroRun.h:502-609.  ostream's default formatting of a
+ * real is printf's %g (precision 6). */
+int ora_save_vtk(ora_run *r, int which, int use_float32, const char *output_dir,
+                 const char *output_prefix, int step) {
+  static const char *const names[4] = {"rho", "E", "mx", "my"}; /* HydroParams.cpp:17 */
+  const ora_params *P = &r->p;
+  const double *U = r->arr[which];
+  char fname[4096];
+  const int one = 1;
+  if (which < 0 || which > 1) return -1;
+  snprintf(fname, sizeof(fname), "%s/%s_%07d.vti", output_dir, output_prefix, step);
+  FILE *f = fopen(fname, "w");
+  if (!f) return -2;
+  fprintf(f, "<?xml version=\"1.0\"?>\n");
+  fprintf(f, "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"%s\">\n",
+          (*(const char *)&one) ? "LittleEndian" : "BigEndian");
+  fprintf(f, "  <ImageData WholeExtent=\"%d %d %d %d %d %d\" Origin=\"%g %g %g\" "
+             "Spacing=\"%g %g %g\">\n",
+          0, P->nx, 0, P->ny, 0, 0, P->xmin, P->ymin, 0.0, P->dx, P->dy, 0.0);
+  fprintf(f, "  <Piece Extent=\"%d %d %d %d %d %d \">\n", 0, P->nx, 0, P->ny, 0, 0);
+  fprintf(f, "    <PointData>\n    </PointData>\n    <CellData>\n");
+  const int gw = P->ghostWidth;
+  for (int v = 0; v < 4; v++) {
+    fprintf(f, "    <DataArray type=\"%s\" Name=\"%s\" format=\"ascii\" >\n",
+            use_float32 ? "Float32" : "Float64", names[v]);
+    for (int index = 0; index < P->isize * P->jsize; ++index) {
+      const int j = index / P->isize;
+      const int i = index - j * P->isize;
+      if (j >= P->jmin + gw && j <= P->jmax - gw && i >= P->imin + gw && i <= P->imax - gw) {
+        const double x = AT(U, i, j, v);
+        fprintf(f, "%g ", use_float32 ? (double)(float)x : x);
+      }
+    }
+    fprintf(f, "\n    </DataArray>\n");
+  }
+  fprintf(f, "    </CellData>\n  </Piece>\n  </ImageData>\n</VTKFile>\n");
+  return fclose(f) == 0 ? 0 : -2;
+}
+
+/* ComputeRadialProfileFunctor, ComputeRadialProfileFunctor.h:83-165 */
+int ora_radial_profile(ora_run *r, int which, int nbins, double *distances, double *density,
+                       int64_t *counts) {
+  const ora_params *P = &r->p;
+  const double *U = r->arr[which];
+  if (which < 0 || which > 1 || nbins < 1) return -1;
+  const double cx = (P->xmin + P->xmax) / 2, cy = (P->ymin + P->ymax) / 2;
+  const double Dx = (P->xmax - P->xmin) / 2, Dy = (P->ymax - P->ymin) / 2;
+  const double maxr = sqrt(Dx * Dx + Dy * Dy);
+  int64_t *cnt = (int64_t *)calloc((size_t)nbins, sizeof(int64_t));
+  double *sum = (double *)calloc((size_t)nbins, sizeof(double));
+  if (!cnt || !sum) {
+    free(cnt);
+    free(sum);
+    return -3;
+  }
+  for (int i = 0; i < P->isize; i++)
+    for (int j = 0; j < P->jsize; j++) {
+      const double x = P->xmin + P->dx / 2 + (i - P->ghostWidth) * P->dx;
+      const double y = P->ymin + P->dy / 2 + (j - P->ghostWidth) * P->dy;
+      const double distance = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy));
+      const int bin = (int)(distance / maxr * nbins);
+      if (bin < 0 || bin >= nbins) continue;
+      cnt[bin] += 1;
+      sum[bin] += AT(U, i, j, ID);
+    }
+  const double dr = maxr / nbins;
+  for (int b = 0; b < nbins; b++) {
+    distances[b] = (b + 0.5) * dr;
+    density[b] = sum[b] / (int)cnt[b];
+    if (counts) counts[b] = cnt[b];
+  }
+  free(cnt);
+  free(sum);
+  return 0;
+}
+
+/* cnpy::create_npy_header<double> + npy_save, cnpy.h:129-187, 343-376 */
+int ora_write_npy(const char *path, const double *data, int64_t n) {
+  char dict[256];
+  const int one = 1;
+  int len = snprintf(dict, sizeof(dict), "{'descr': '%cf8', 'fortran_order': False, 'shape': (%lld,), }",
+                     (*(const char *)&one) ? '<' : '>', (long long)n);
+  const int pad = 16 - (10 + len) % 16;
+  for (int k = 0; k < pad; k++) dict[len + k] = ' ';
+  len += pad;
+  dict[len - 1] = '\n';
+  FILE *f = fopen(path, "wb");
+  if (!f) return -2;
+  const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+  const unsigned char hl[2] = {(unsigned char)(len & 0xff), (unsigned char)(len >> 8)};
+  int ok = fwrite(magic, 1, 8, f) == 8 && fwrite(hl, 1, 2, f) == 2 &&
+           fwrite(dict, 1, (size_t)len, f) == (size_t)len &&
+           (n == 0 || fwrite(data, sizeof(double), (size_t)n, f) == (size_t)n);
+  return (fclose(f) == 0 && ok) ? 0 : -2;
 }
 
 static void set4(double *U, const ora_params *P, int i, int j, double d, double e, double mx, double my) {

@@ -111,6 +111,17 @@ void ora_run_loop(ora_run *r, double *t, int *nStep, int max_steps);
 int ora_get_array(ora_run *r, int which, double *out);
 int ora_set_array(ora_run *r, int which, const double *in);
 
+/* output formats */
+/* HydroRun::saveVTK (HydroRun.h:502-609) of array `which` (0 U, 1 U2) */
+int ora_save_vtk(ora_run *r, int which, int use_float32, const char *output_dir,
+                 const char *output_prefix, int step);
+/* ComputeRadialProfileFunctor::apply (ComputeRadialProfileFunctor.h:83-165)
+ * of array `which`, sequential sums; cells past the last bin are skipped */
+int ora_radial_profile(ora_run *r, int which, int nbins, double *distances, double *density,
+                       int64_t *counts);
+/* cnpy::npy_save of a 1-D double array (cnpy.h:129-187, 343-376) */
+int ora_write_npy(const char *path, const double *data, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -252,3 +252,43 @@ def ref_config_lib():
     L.refcfg_get_bool.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]
     _ref = L
     return L
+
+
+def _output_protos() -> None:
+    L = lib()
+    if getattr(L, "_output_ready", False):
+        return
+    L.ora_save_vtk.restype = C.c_int
+    L.ora_save_vtk.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int]
+    L.ora_radial_profile.restype = C.c_int
+    L.ora_radial_profile.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
+    L.ora_write_npy.restype = C.c_int
+    L.ora_write_npy.argtypes = [C.c_char_p, C.c_void_p, C.c_int64]
+    L._output_ready = True
+
+
+def save_vtk(run: "OracleRun", which: int, output_dir, prefix: str, step: int,
+             float32: bool = False) -> Path:
+    _output_protos()
+    rc = lib().ora_save_vtk(run._h, which, int(float32), str(output_dir).encode(),
+                            prefix.encode(), step)
+    assert rc == 0, rc
+    return Path(output_dir) / f"{prefix}_{step:07d}.vti"
+
+
+def radial_profile(run: "OracleRun", which: int, nbins: int):
+    _output_protos()
+    d = np.empty(nbins)
+    m = np.empty(nbins)
+    c = np.empty(nbins, dtype=np.int64)
+    rc = lib().ora_radial_profile(run._h, which, nbins, d.ctypes.data, m.ctypes.data,
+                                  c.ctypes.data)
+    assert rc == 0, rc
+    return d, m, c
+
+
+def write_npy(path, data: np.ndarray) -> None:
+    _output_protos()
+    a = np.ascontiguousarray(data, dtype=np.float64).ravel()
+    assert lib().ora_write_npy(str(path).encode(), a.ctypes.data, a.size) == 0
