@@ -45,6 +45,23 @@ DECKS = {  # deck, description, steps of the workload
 TILES = (8, 10, 12, 14, 16, 20, 24, 32, 48, 64)
 
 
+PROFILE = ROOT / "profiles" / "step_fused_fp64_blast_1024x1536.md"
+
+
+def profiled_traffic(workload: str, tile: int):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary, when it covers this workload and tile."""
+    import re
+    if workload != "blast" or not PROFILE.exists():
+        return None
+    text = PROFILE.read_text()
+    m = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
+    t = re.search(r"seg_rows (\d+)", text)
+    if not m or not t or int(t.group(1)) != tile:
+        return None
+    return int(m.group(1))
+
+
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,7 +247,7 @@ def main() -> int:
                    "tile": run.get_tile(), "device": dev, "cus": ncu},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": profiled_traffic(args.workload, run.get_tile())},
         "kernel_ms_per_step": round(per_step_ms, 5),
         "final_t": run.t,
     }
