@@ -870,6 +870,7 @@ struct e2d_batch {
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
   int graph_tile = 0;
+  int floor_sqrt = 1;  // every instance allows the short wave-speed square root
 };
 
 namespace {
@@ -915,8 +916,10 @@ int batch_upload_table(e2d_batch *b) {
 }
 
 int batch_launch(e2d_batch *b) {
-  HIP_TRY(e2d::launch_step_batch(b->runs[0]->kp, b->insts, (int)b->runs.size(), b->ctls,
-                                 b->nwaves, b->seg_rows, b->blocks_done, b->stream));
+  e2d::KParams k0 = b->runs[0]->kp;
+  k0.s.floor_sqrt = b->floor_sqrt;
+  HIP_TRY(e2d::launch_step_batch(k0, b->insts, (int)b->runs.size(), b->ctls, b->nwaves,
+                                 b->seg_rows, b->blocks_done, b->stream));
   return E2D_OK;
 }
 
@@ -972,6 +975,7 @@ int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_bat
         return fail(rc, msg);
       }
       b->runs.push_back(r);
+      b->floor_sqrt = b->floor_sqrt && r->kp.s.floor_sqrt;
       if (r->kp.s.riemann != b->runs[0]->kp.s.riemann) {
         e2d_batch_destroy(b);
         return fail(E2D_ERR_UNSUPPORTED, "all instances of a batch must use the same Riemann solver");

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -1021,6 +1022,15 @@ void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
 template <class P>
 void launch_fused_p(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl, int seg_rows,
                     hipStream_t st) {
+  if constexpr (std::is_same<P, P64>::value) {
+    if (kp.s.floor_sqrt && kp.s.riemann != 0) {  // HLL / HLLC with the short square root
+      if (kp.s.riemann == 2)
+        launch_fused_t<2 | kRiemannFloorSqrt, P>(kp, in, out, dt, ctl, seg_rows, st);
+      else
+        launch_fused_t<1 | kRiemannFloorSqrt, P>(kp, in, out, dt, ctl, seg_rows, st);
+      return;
+    }
+  }
   if (kp.s.riemann == 2)
     launch_fused_t<2, P>(kp, in, out, dt, ctl, seg_rows, st);
   else if (kp.s.riemann == 1)
@@ -1045,6 +1055,7 @@ KParams make_kparams(const e2d_params &p, int precision) {
   k.s.slope_type = p.slope_type;
   k.s.entho = 1.0 / (p.gamma0 - 1.0);
   k.s.smallc2 = p.smallc * p.smallc;
+  k.s.floor_sqrt = k.s.smallc2 >= std::ldexp(1.0, -759) ? 1 : 0;
   k.s.riemann = p.honor_riemann ? p.riemannSolverType : 2;
   k.sf.gamma0 = (float)k.s.gamma0;
   k.sf.gamma6 = (float)k.s.gamma6;
@@ -1055,6 +1066,7 @@ KParams make_kparams(const e2d_params &p, int precision) {
   k.sf.slope_type = (float)k.s.slope_type;
   k.sf.entho = (float)k.s.entho;
   k.sf.smallc2 = (float)k.s.smallc2;
+  k.sf.floor_sqrt = 0;
   k.sf.riemann = k.s.riemann;
   k.prec = precision;
   k.nx = p.nx;
@@ -1188,6 +1200,9 @@ hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int
     if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, P32);
     else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, P32);
     else E2D_BATCH_LAUNCH(0, P32);
+  } else if (kp0.s.floor_sqrt && kp0.s.riemann != 0) {
+    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2 | kRiemannFloorSqrt, P64);
+    else E2D_BATCH_LAUNCH(1 | kRiemannFloorSqrt, P64);
   } else {
     if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, P64);
     else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, P64);

@@ -97,6 +97,7 @@ int sums_partials_needed(const KParams &kp);
 // wavefronts the fused step launches for one grid (and its strip count)
 int batch_waves(const KParams &kp, int seg_rows, int *nstrips);
 // one step of every active instance; kp0 gives the precision and solver of all
+// (kp0.s.floor_sqrt must hold for every instance)
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
                              StepCtl *ctls_dev, int nwaves, int seg_rows,
                              unsigned int *blocks_done, hipStream_t st);

@@ -33,6 +33,28 @@ struct Ops<double> {
   static __device__ __forceinline__ double ab(double a) { return fabs(a); }
 };
 
+// Square root of x >= smallc^2 for the wave-speed estimates.  The compiler's
+// correctly rounded FP64 sqrt first scales x < 2^-767 by 2^256, then runs
+// rsq + one Goldschmidt step + two Newton corrections, unscales, and returns
+// x itself for +-0 and +inf.  For x in [2^-767, DBL_MAX] the scaling and the
+// special cases are identities, so the same core sequence, operation for
+// operation, gives the same bits; the host enables it only when smallc^2 (the
+// floor of every argument) is >= 2^-759.  A state with infinite pressure gets
+// NaN here where sqrt returns inf.
+__device__ __forceinline__ double sqrt_core(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  double d = __builtin_fma(-g, g, x);
+  h = __builtin_fma(h, r, h);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return g;
+}
+
 template <>
 struct Ops<float> {
   static __device__ __forceinline__ float div(float a, float b) {
@@ -52,8 +74,19 @@ struct SchemeT {
   T gamma0, gamma6, smallr, smallc, smallp, smallpp, slope_type, entho;
   T smallc2;    // smallc * smallc
   int riemann;  // 0 approx, 1 hll, 2 hllc (already resolved from honor_riemann)
+  int floor_sqrt;  // smallc2 >= 2^-759: floored sound-speed roots take the short sqrt
 };
 using Scheme = SchemeT<double>;
+
+// sqrt(x) for x >= s.smallc2 (the wave-speed estimates of the Riemann
+// solvers); FS selects the short sequence (FP64 only, when s.floor_sqrt)
+template <bool FS, class T>
+__device__ __forceinline__ T floored_sqrt(T x) {
+  if constexpr (FS)
+    return sqrt_core(x);
+  else
+    return Ops<T>::sqroot(x);
+}
 
 template <class T>
 struct Q4T {
@@ -193,7 +226,7 @@ __device__ __forceinline__ void trace_face(const SchemeT<T> &s, const Q4T<T> &q,
 // The reference evaluates both star states and then picks one; here the side
 // (left if ustar > 0) is picked first and its star state evaluated once with
 // the same expressions, which gives identical bits for the state it keeps.
-template <class T>
+template <bool FS = false, class T>
 __device__ __forceinline__ void hllc(const SchemeT<T> &s, const Q4T<T> &ql, const Q4T<T> &qr,
                                      int ln, int tn, T flux[4]) {
   using O = Ops<T>;
@@ -214,8 +247,8 @@ __device__ __forceinline__ void hllc(const SchemeT<T> &s, const Q4T<T> &ql, cons
   const T etotr = pr * s.entho + ecinr;
   const T ptotr = pr;
 
-  const T cfastl = O::sqroot(O::mx(O::div(s.gamma0 * pl, rl), s.smallc2));
-  const T cfastr = O::sqroot(O::mx(O::div(s.gamma0 * pr, rr), s.smallc2));
+  const T cfastl = floored_sqrt<FS>(O::mx(O::div(s.gamma0 * pl, rl), s.smallc2));
+  const T cfastr = floored_sqrt<FS>(O::mx(O::div(s.gamma0 * pr, rr), s.smallc2));
   const T cmax = O::mx(cfastl, cfastr);
   const T SL = O::mn(ul, ur) - cmax;
   const T SR = O::mx(ul, ur) + cmax;
@@ -252,7 +285,7 @@ __device__ __forceinline__ void hllc(const SchemeT<T> &s, const Q4T<T> &ql, cons
 // HLL (no reference counterpart; the reference declares the enum only,
 // HydroParams.h:49).  Toro 2009, section 10.3, eq. (10.21), with the floors and
 // wave speeds of riemann_hllc.  Same expressions as the CPU oracle.
-template <class T>
+template <bool FS = false, class T>
 __device__ __forceinline__ void hll(const SchemeT<T> &s, const Q4T<T> &ql, const Q4T<T> &qr,
                                     int ln, int tn, T flux[4]) {
   using O = Ops<T>;
@@ -265,8 +298,8 @@ __device__ __forceinline__ void hll(const SchemeT<T> &s, const Q4T<T> &ql, const
   const T ur = qr.v[ln], vr = qr.v[tn];
   const T el = pl * s.entho + (h * rl * ul * ul + h * rl * vl * vl);
   const T er = pr * s.entho + (h * rr * ur * ur + h * rr * vr * vr);
-  const T cfastl = O::sqroot(O::mx(O::div(s.gamma0 * pl, rl), s.smallc2));
-  const T cfastr = O::sqroot(O::mx(O::div(s.gamma0 * pr, rr), s.smallc2));
+  const T cfastl = floored_sqrt<FS>(O::mx(O::div(s.gamma0 * pl, rl), s.smallc2));
+  const T cfastr = floored_sqrt<FS>(O::mx(O::div(s.gamma0 * pr, rr), s.smallc2));
   const T SL = O::mn(ul, ur) - O::mx(cfastl, cfastr);
   const T SR = O::mx(ul, ur) + O::mx(cfastl, cfastr);
   // variable order of the conserved state: rho, E, normal momentum, transverse momentum
@@ -368,13 +401,19 @@ __device__ __forceinline__ void approx(const SchemeT<T> &s, const Q4T<T> &qlf, c
 }
 
 // solver chosen at compile time: RS = 0 approx, 1 hll, 2 hllc
+// RS: 0 approx, 1 HLL, 2 HLLC; + kRiemannFloorSqrt: the HLL/HLLC wave speeds
+// use the short square root (FP64, arguments floored at smallc^2 >= 2^-759)
+constexpr int kRiemannFloorSqrt = 8;
+
 template <int RS, class T>
 __device__ __forceinline__ void riemann(const SchemeT<T> &s, const Q4T<T> &ql, const Q4T<T> &qr,
                                         int ln, int tn, T flux[4]) {
-  if constexpr (RS == 2)
-    hllc(s, ql, qr, ln, tn, flux);
-  else if constexpr (RS == 1)
-    hll(s, ql, qr, ln, tn, flux);
+  constexpr int kind = RS & 7;
+  constexpr bool fs = (RS & kRiemannFloorSqrt) != 0;
+  if constexpr (kind == 2)
+    hllc<fs>(s, ql, qr, ln, tn, flux);
+  else if constexpr (kind == 1)
+    hll<fs>(s, ql, qr, ln, tn, flux);
   else
     approx(s, ql, qr, ln, tn, flux);
 }

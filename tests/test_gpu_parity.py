@@ -55,6 +55,9 @@ CASES = {
     "sedov_energy": dict(nx=50, ny=50, problemType=1, cfl=0.7, nStepmax=30, tEnd=1e9,
                          blast_total_energy_inside=1.0, blast_radius=0.05,
                          blast_density_out=1.0),
+    # smallc^2 below 2^-759: the wave speeds take the full square root
+    "blast_tiny_smallc": dict(nx=40, ny=48, problemType=1, cfl=0.7, nStepmax=30, tEnd=1e9,
+                              smallc=1e-200, blast_radius=0.15),
 }
 
 
