@@ -32,6 +32,8 @@ E2D_PRECISION_FP32 = 2
 
 E2D_SCHEDULE_FUSED = 0
 E2D_SCHEDULE_PASSES = 1
+E2D_SCHEDULE_DIRECTIONAL = 2
+E2D_SCHEDULE_ATOMIC = 3
 
 _I = C.c_int32
 _D = C.c_double
@@ -111,6 +113,7 @@ _PROTOTYPES = {
     "e2d_copy_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "e2d_conserved_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "e2d_timers": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "e2d_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "e2d_save_vtk": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
     "e2d_write_vti": (C.c_int, [C.POINTER(E2DParams), C.c_void_p, C.c_int, C.c_char_p,
                                 C.c_char_p, C.c_int]),

@@ -60,7 +60,7 @@ struct e2d_run {
   int seg_rows = 16;
   int device = 0;
   hipStream_t stream = nullptr;
-  Grid U{}, U2{}, Q{}, Fx{}, Fy{};
+  Grid U{}, U2{}, Q{}, Fx{}, Fy{}, Sx{}, Sy{};
   e2d::StepCtl *ctl = nullptr;
   unsigned long long *bits = nullptr;
   unsigned int *count = nullptr;
@@ -109,14 +109,26 @@ int check_run(e2d_run *r) {
 }
 
 int ensure_pass_arrays(e2d_run *r) {
-  if (r->Q.p) return E2D_OK;
-  int rc = alloc_grid(r, r->Q, E2D_PRECISION_FP64);
-  if (!rc) rc = alloc_grid(r, r->Fx, E2D_PRECISION_FP64);
-  if (!rc) rc = alloc_grid(r, r->Fy, E2D_PRECISION_FP64);
-  if (!rc) {
-    const size_t bytes = (size_t)r->Fx.plane * 4 * sizeof(double);
-    HIP_TRY(hipMemsetAsync(r->Fx.p, 0, bytes, r->stream));
-    HIP_TRY(hipMemsetAsync(r->Fy.p, 0, bytes, r->stream));
+  int rc = E2D_OK;
+  if (!r->Q.p) {
+    rc = alloc_grid(r, r->Q, E2D_PRECISION_FP64);
+    if (!rc) rc = alloc_grid(r, r->Fx, E2D_PRECISION_FP64);
+    if (!rc) rc = alloc_grid(r, r->Fy, E2D_PRECISION_FP64);
+    if (!rc) {
+      const size_t bytes = (size_t)r->Fx.plane * 4 * sizeof(double);
+      HIP_TRY(hipMemsetAsync(r->Fx.p, 0, bytes, r->stream));
+      HIP_TRY(hipMemsetAsync(r->Fy.p, 0, bytes, r->stream));
+    }
+  }
+  if (!rc && r->schedule == E2D_SCHEDULE_DIRECTIONAL && !r->Sx.p) {
+    // Slopes_x / Slopes_y of implementation version 1 (HydroRun.h:168-172)
+    rc = alloc_grid(r, r->Sx, E2D_PRECISION_FP64);
+    if (!rc) rc = alloc_grid(r, r->Sy, E2D_PRECISION_FP64);
+    if (!rc) {
+      const size_t bytes = (size_t)r->Sx.plane * 4 * sizeof(double);
+      HIP_TRY(hipMemsetAsync(r->Sx.p, 0, bytes, r->stream));
+      HIP_TRY(hipMemsetAsync(r->Sy.p, 0, bytes, r->stream));
+    }
   }
   return rc;
 }
@@ -130,20 +142,33 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
   hipEvent_t ev[5] = {};
   if (timed)
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-  if (r->schedule == E2D_SCHEDULE_PASSES) {
+  if (r->schedule != E2D_SCHEDULE_FUSED) {
     if (r->precision != E2D_PRECISION_FP64)
-      return fail(E2D_ERR_UNSUPPORTED, "the pass schedule exists for FP64 only");
+      return fail(E2D_ERR_UNSUPPORTED, "the pass schedules exist for FP64 only");
     int rc = ensure_pass_arrays(r);
     if (rc) return rc;
+    // HydroRun::godunov_unsplit_impl, HydroRun.h:279-364
     if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
     HIP_TRY(e2d::launch_boundaries(r->kp, in, r->stream));
     HIP_TRY(e2d::launch_copy(r->kp, in, out, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[1], r->stream));
     HIP_TRY(e2d::launch_prim(r->kp, in, r->Q, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[2], r->stream));
-    HIP_TRY(e2d::launch_fluxes(r->kp, r->Q, r->Fx, r->Fy, dtdx, dtdy, r->stream));
-    if (timed) HIP_TRY(hipEventRecord(ev[3], r->stream));
-    HIP_TRY(e2d::launch_update(r->kp, out, r->Fx, r->Fy, r->stream));
+    if (r->schedule == E2D_SCHEDULE_PASSES) {
+      HIP_TRY(e2d::launch_fluxes(r->kp, r->Q, r->Fx, r->Fy, dtdx, dtdy, r->stream));
+      if (timed) HIP_TRY(hipEventRecord(ev[3], r->stream));
+      HIP_TRY(e2d::launch_update(r->kp, out, r->Fx, r->Fy, r->stream));
+    } else if (r->schedule == E2D_SCHEDULE_DIRECTIONAL) {
+      HIP_TRY(e2d::launch_slopes(r->kp, r->Q, r->Sx, r->Sy, r->stream));
+      HIP_TRY(e2d::launch_trace_fluxes(r->kp, 0, r->Q, r->Sx, r->Sy, r->Fx, dtdx, dtdy,
+                                       r->stream));
+      HIP_TRY(e2d::launch_update_dir(r->kp, 0, out, r->Fx, r->stream));
+      HIP_TRY(e2d::launch_trace_fluxes(r->kp, 1, r->Q, r->Sx, r->Sy, r->Fy, dtdx, dtdy,
+                                       r->stream));
+      HIP_TRY(e2d::launch_update_dir(r->kp, 1, out, r->Fy, r->stream));
+    } else {
+      HIP_TRY(e2d::launch_fluxes_atomic(r->kp, r->Q, out, dtdx, dtdy, r->stream));
+    }
     if (timed) HIP_TRY(hipEventRecord(ev[4], r->stream));
   } else {
     if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
@@ -153,15 +178,17 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
   if (timed) {
     HIP_TRY(hipEventSynchronize(ev[4]));
     float ms = 0.f;
-    if (r->schedule == E2D_SCHEDULE_PASSES) {
+    if (r->schedule != E2D_SCHEDULE_FUSED) {
       HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
       r->timers[0] += ms * 1e-3;
       HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2]));
       r->timers[2] += ms * 1e-3;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3]));
-      r->timers[3] += ms * 1e-3;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[3], ev[4]));
-      r->timers[4] += ms * 1e-3;
+      if (r->schedule == E2D_SCHEDULE_PASSES) {  // flux / update timers of version 0 only
+        HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        r->timers[3] += ms * 1e-3;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[3], ev[4]));
+        r->timers[4] += ms * 1e-3;
+      }
       HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[4]));
       r->timers[1] += ms * 1e-3;
     } else {
@@ -205,7 +232,7 @@ int loop_read(e2d_run *r, e2d::StepCtl &c) {
 int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elapsed_ms,
                   int *executed) {
   if (!t || !nStep) return fail(E2D_ERR_ARG, "null t / nStep");
-  if (r->schedule == E2D_SCHEDULE_PASSES) {
+  if (r->schedule != E2D_SCHEDULE_FUSED) {
     // reference structure: host dt per step, exactly main.cpp:100-145
     const int n0 = *nStep;
     if (*nStep == 0) {
@@ -533,6 +560,12 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
     r->p = *p;
     r->kp = e2d::make_kparams(*p, precision);
     r->precision = precision;
+    // implementationVersion selects the schedule, as in godunov_unsplit_impl
+    // (HydroRun.h:312-357); version 0 runs fused, with the same bits
+    if (precision == E2D_PRECISION_FP64 && p->implementationVersion == 1)
+      r->schedule = E2D_SCHEDULE_DIRECTIONAL;
+    else if (precision == E2D_PRECISION_FP64 && p->implementationVersion == 2)
+      r->schedule = E2D_SCHEDULE_ATOMIC;
     HIP_TRY(hipGetDevice(&r->device));
     e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -768,9 +801,15 @@ int e2d_timers(e2d_run *r, double out[5]) {
 
 int e2d_set_schedule(e2d_run *r, int schedule) {
   if (!r) return fail(E2D_ERR_ARG, "null e2d_run handle");
-  if (schedule != E2D_SCHEDULE_FUSED && schedule != E2D_SCHEDULE_PASSES)
+  if (schedule < E2D_SCHEDULE_FUSED || schedule > E2D_SCHEDULE_ATOMIC)
     return fail(E2D_ERR_ARG, "unknown schedule");
   r->schedule = schedule;
+  return E2D_OK;
+}
+
+int e2d_get_schedule(e2d_run *r, int *schedule) {
+  if (!r || !schedule) return fail(E2D_ERR_ARG, "null argument");
+  *schedule = r->schedule;
   return E2D_OK;
 }
 

@@ -556,6 +556,105 @@ __global__ void update_kernel(KParams kp, Grid U, Grid Fx, Grid Fy) {
   }
 }
 
+// ---------------------------------- implementation version 1 (directional)
+// ComputeSlopesFunctor (HydroRunFunctors.h:1061-1162): slopes stored once,
+// for i, j in [1, size-2]
+__global__ void slopes_kernel(KParams kp, Grid Q, Grid Sx, Grid Sy) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > kp.isize - 2 || j > kp.jsize - 2) return;
+  Q4 dqX, dqY;
+  slopes(kp.s, ld4(Q, i, j), ld4(Q, i + 1, j), ld4(Q, i - 1, j), ld4(Q, i, j + 1),
+         ld4(Q, i, j - 1), dqX, dqY);
+  for (int v = 0; v < 4; v++) {
+    at(Sx, v, i, j) = dqX.v[v];
+    at(Sy, v, i, j) = dqY.v[v];
+  }
+}
+
+// ComputeTraceAndFluxes_Functor<dir> (:1167-1342): the flux through the
+// left (DIR 0) or bottom (DIR 1) face of (i, j), i, j in [2, size-2]
+template <int RS, int DIR>
+__global__ void trace_fluxes_kernel(KParams kp, Grid Q, Grid Sx, Grid Sy, Grid F, double dtdx,
+                                    double dtdy) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > kp.isize - 2 || j > kp.jsize - 2) return;
+  const Scheme &s = kp.s;
+  const int in = DIR == 0 ? i - 1 : i, jn = DIR == 0 ? j : j - 1;
+  const Q4 qc = ld4(Q, i, j), qn = ld4(Q, in, jn);
+  const Q4 dqX = ld4(Sx, i, j), dqY = ld4(Sy, i, j);
+  const Q4 dqXn = ld4(Sx, in, jn), dqYn = ld4(Sy, in, jn);
+  const Sources sc = trace_sources(s, qc, dqX, dqY);
+  const Sources sn = trace_sources(s, qn, dqXn, dqYn);
+  const double dtd = DIR == 0 ? dtdx : dtdy;
+  Q4 fr, fl;
+  trace_face(s, qc, DIR == 0 ? dqX : dqY, sc, dtd, false, fr);
+  trace_face(s, qn, DIR == 0 ? dqXn : dqYn, sn, dtd, true, fl);
+  double flux[4];
+  riemann<RS>(s, fl, fr, DIR == 0 ? IU : IV, DIR == 0 ? IV : IU, flux);
+  for (int v = 0; v < 4; v++) at(F, v, i, j) = flux[v] * dtd;
+}
+
+// UpdateDirFunctor<dir> (:986-1055): U += F(face before); U -= F(face after)
+template <int DIR>
+__global__ void update_dir_kernel(KParams kp, Grid U, Grid F) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > kp.nx + 1 || j > kp.ny + 1) return;
+  for (int v = 0; v < 4; v++) {
+    double u = at(U, v, i, j);
+    u += at(F, v, i, j);
+    u -= DIR == 0 ? at(F, v, i + 1, j) : at(F, v, i, j + 1);
+    at(U, v, i, j) = u;
+  }
+}
+
+// ---------------------------------- implementation version 2 (atomics)
+// ComputeFluxesAndUpdateFunctor (:728-980): the left and bottom face fluxes of
+// (i, j), i, j in [2, size-2], added to the cells on both sides with FP64
+// atomics.  The order of the additions into a cell is not fixed, so the last
+// bits of the result vary from run to run, as in the reference.
+template <int RS>
+__global__ void fluxes_atomic_kernel(KParams kp, Grid Q, Grid U, double dtdx, double dtdy) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > kp.isize - 2 || j > kp.jsize - 2) return;
+  const Scheme &s = kp.s;
+  const Q4 qc = ld4(Q, i, j);
+  Q4 dqX, dqY, dqXn, dqYn, fr, fl;
+  slopes(s, qc, ld4(Q, i + 1, j), ld4(Q, i - 1, j), ld4(Q, i, j + 1), ld4(Q, i, j - 1), dqX, dqY);
+  const Sources sc = trace_sources(s, qc, dqX, dqY);
+  double flux[4];
+  const bool inner_i = i < kp.isize - 2, inner_j = j < kp.jsize - 2;
+  {
+    const Q4 qn = ld4(Q, i - 1, j);
+    slopes(s, qn, qc, ld4(Q, i - 2, j), ld4(Q, i - 1, j + 1), ld4(Q, i - 1, j - 1), dqXn, dqYn);
+    const Sources sn = trace_sources(s, qn, dqXn, dqYn);
+    trace_face(s, qc, dqX, sc, dtdx, false, fr);
+    trace_face(s, qn, dqXn, sn, dtdx, true, fl);
+    riemann<RS>(s, fl, fr, IU, IV, flux);
+    for (int v = 0; v < 4; v++) {
+      const double f = flux[v] * dtdx;
+      if (inner_j && inner_i) unsafeAtomicAdd(&at(U, v, i, j), f);
+      if (inner_j && i > 2) unsafeAtomicAdd(&at(U, v, i - 1, j), -f);
+    }
+  }
+  {
+    const Q4 qn = ld4(Q, i, j - 1);
+    slopes(s, qn, ld4(Q, i + 1, j - 1), ld4(Q, i - 1, j - 1), qc, ld4(Q, i, j - 2), dqXn, dqYn);
+    const Sources sn = trace_sources(s, qn, dqXn, dqYn);
+    trace_face(s, qc, dqY, sc, dtdy, false, fr);
+    trace_face(s, qn, dqYn, sn, dtdy, true, fl);
+    riemann<RS>(s, fl, fr, IV, IU, flux);
+    for (int v = 0; v < 4; v++) {
+      const double f = flux[v] * dtdy;
+      if (inner_j && inner_i) unsafeAtomicAdd(&at(U, v, i, j), f);
+      if (j > 2 && inner_i) unsafeAtomicAdd(&at(U, v, i, j - 1), -f);
+    }
+  }
+}
+
 // ------------------------------------------------------ fused step kernel
 //
 // One wavefront owns a strip of 64 columns (60 output columns + 2 halo columns
@@ -1164,6 +1263,54 @@ hipError_t launch_fluxes(const KParams &kp, Grid Q, Grid Fx, Grid Fy, double dtd
 hipError_t launch_update(const KParams &kp, Grid U, Grid Fx, Grid Fy, hipStream_t st) {
   const dim3 blk(64, 4);
   hipLaunchKernelGGL(update_kernel, grid2d(kp.nx, kp.ny, blk), blk, 0, st, kp, U, Fx, Fy);
+  return hipGetLastError();
+}
+
+hipError_t launch_slopes(const KParams &kp, Grid Q, Grid Sx, Grid Sy, hipStream_t st) {
+  const dim3 blk(64, 4);
+  hipLaunchKernelGGL(slopes_kernel, grid2d(kp.isize - 2, kp.jsize - 2, blk), blk, 0, st, kp, Q, Sx,
+                     Sy);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_fluxes(const KParams &kp, int dir, Grid Q, Grid Sx, Grid Sy, Grid F,
+                               double dtdx, double dtdy, hipStream_t st) {
+  const dim3 blk(64, 4);
+  const dim3 grd = grid2d(kp.nx + 1, kp.ny + 1, blk);
+#define E2D_TF(RS, D) \
+  hipLaunchKernelGGL((trace_fluxes_kernel<RS, D>), grd, blk, 0, st, kp, Q, Sx, Sy, F, dtdx, dtdy)
+  if (dir == 0) {
+    if (kp.s.riemann == 2) E2D_TF(2, 0);
+    else if (kp.s.riemann == 1) E2D_TF(1, 0);
+    else E2D_TF(0, 0);
+  } else {
+    if (kp.s.riemann == 2) E2D_TF(2, 1);
+    else if (kp.s.riemann == 1) E2D_TF(1, 1);
+    else E2D_TF(0, 1);
+  }
+#undef E2D_TF
+  return hipGetLastError();
+}
+
+hipError_t launch_update_dir(const KParams &kp, int dir, Grid U, Grid F, hipStream_t st) {
+  const dim3 blk(64, 4);
+  if (dir == 0)
+    hipLaunchKernelGGL(update_dir_kernel<0>, grid2d(kp.nx, kp.ny, blk), blk, 0, st, kp, U, F);
+  else
+    hipLaunchKernelGGL(update_dir_kernel<1>, grid2d(kp.nx, kp.ny, blk), blk, 0, st, kp, U, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_fluxes_atomic(const KParams &kp, Grid Q, Grid U, double dtdx, double dtdy,
+                                hipStream_t st) {
+  const dim3 blk(64, 4);
+  const dim3 grd = grid2d(kp.nx + 1, kp.ny + 1, blk);
+  if (kp.s.riemann == 2)
+    hipLaunchKernelGGL(fluxes_atomic_kernel<2>, grd, blk, 0, st, kp, Q, U, dtdx, dtdy);
+  else if (kp.s.riemann == 1)
+    hipLaunchKernelGGL(fluxes_atomic_kernel<1>, grd, blk, 0, st, kp, Q, U, dtdx, dtdy);
+  else
+    hipLaunchKernelGGL(fluxes_atomic_kernel<0>, grd, blk, 0, st, kp, Q, U, dtdx, dtdy);
   return hipGetLastError();
 }
 

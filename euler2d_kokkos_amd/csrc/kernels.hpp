@@ -89,6 +89,15 @@ hipError_t launch_prim(const KParams &kp, Grid U, Grid Q, hipStream_t st);
 hipError_t launch_fluxes(const KParams &kp, Grid Q, Grid Fx, Grid Fy, double dtdx, double dtdy,
                          hipStream_t st);
 hipError_t launch_update(const KParams &kp, Grid U, Grid Fx, Grid Fy, hipStream_t st);
+// implementation version 1: stored slopes, then trace + flux and update per
+// direction (dir 0 = x, 1 = y), FP64 only
+hipError_t launch_slopes(const KParams &kp, Grid Q, Grid Sx, Grid Sy, hipStream_t st);
+hipError_t launch_trace_fluxes(const KParams &kp, int dir, Grid Q, Grid Sx, Grid Sy, Grid F,
+                               double dtdx, double dtdy, hipStream_t st);
+hipError_t launch_update_dir(const KParams &kp, int dir, Grid U, Grid F, hipStream_t st);
+// implementation version 2: fluxes added to both neighbours with FP64 atomics
+hipError_t launch_fluxes_atomic(const KParams &kp, Grid Q, Grid U, double dtdx, double dtdy,
+                                hipStream_t st);
 // ctl == nullptr: explicit dt (HydroRun::godunov_unsplit); else run-loop mode
 hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl,
                              int seg_rows, hipStream_t st);

@@ -24,12 +24,16 @@ static int die(const char *what) {
 
 int main(int argc, char *argv[]) {
   const char *ini = nullptr;
-  int schedule = E2D_SCHEDULE_FUSED;
+  int schedule = -1;  // -1: the one implementationVersion selects
   for (int a = 1; a < argc; ++a) {
     if (!std::strcmp(argv[a], "--schedule=passes"))
       schedule = E2D_SCHEDULE_PASSES;
     else if (!std::strcmp(argv[a], "--schedule=fused"))
       schedule = E2D_SCHEDULE_FUSED;
+    else if (!std::strcmp(argv[a], "--schedule=directional"))
+      schedule = E2D_SCHEDULE_DIRECTIONAL;
+    else if (!std::strcmp(argv[a], "--schedule=atomic"))
+      schedule = E2D_SCHEDULE_ATOMIC;
     else if (!ini)
       ini = argv[a];
     else
@@ -72,7 +76,7 @@ int main(int argc, char *argv[]) {
 
   e2d_run *run = nullptr;
   if (e2d_create(&params, E2D_PRECISION_FP64, &run)) return die("create");
-  if (e2d_set_schedule(run, schedule)) return die("schedule");
+  if (schedule >= 0 && e2d_set_schedule(run, schedule)) return die("schedule");
   if (e2d_init_problem(run)) return die("init");
 
   const bool output = params.enableOutput && params.nOutput > 0;

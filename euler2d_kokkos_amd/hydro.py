@@ -120,12 +120,14 @@ class HydroRun:
     """
 
     def __init__(self, params: HydroParams, precision: int = _capi.E2D_PRECISION_FP64,
-                 schedule: int = _capi.E2D_SCHEDULE_FUSED, initialise: bool = True):
+                 schedule: int | None = None, initialise: bool = True):
+        """schedule None keeps the one implementationVersion selects."""
         lib = load_library()
         self.params = params
         self._h = C.c_void_p()
         check(lib.e2d_create(C.byref(params.raw), precision, C.byref(self._h)))
-        check(lib.e2d_set_schedule(self._h, schedule))
+        if schedule is not None:
+            check(lib.e2d_set_schedule(self._h, schedule))
         self.t = 0.0
         self.nStep = 0
         if initialise:
@@ -163,6 +165,11 @@ class HydroRun:
 
     def set_schedule(self, schedule: int) -> None:
         check(load_library().e2d_set_schedule(self._h, schedule))
+
+    def get_schedule(self) -> int:
+        out = C.c_int(-1)
+        check(load_library().e2d_get_schedule(self._h, C.byref(out)))
+        return out.value
 
     # -- main.cpp time loop with device-resident dt
     def run(self, max_steps: int = 0) -> tuple[float, int]:

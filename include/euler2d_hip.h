@@ -57,6 +57,10 @@ extern "C" {
 /* step schedules of the FP64 path */
 #define E2D_SCHEDULE_FUSED 0    /* one fused kernel per step (default) */
 #define E2D_SCHEDULE_PASSES 1   /* reference pass structure: BC, copy, prim, fluxes, update */
+#define E2D_SCHEDULE_DIRECTIONAL 2 /* implementationVersion 1: stored slopes, x then y
+                                      trace/flux + update passes (HydroRun.h:334-351) */
+#define E2D_SCHEDULE_ATOMIC 3   /* implementationVersion 2: fluxes added to both neighbours
+                                   with FP64 atomics (HydroRun.h:352-357); last bits vary */
 
 /*
  * POD mirror of HydroParams + HydroSettings + blast + shocked-bubble fields
@@ -184,7 +188,11 @@ int e2d_conserved_sums(e2d_run *run, int which, double sums[4]);
  * boundaries, godunov, primitives, fluxes, update (the fused schedule books
  * its whole step under godunov) */
 int e2d_timers(e2d_run *run, double out[5]);
+/* E2D_SCHEDULE_*.  e2d_create picks the schedule of the params'
+ * implementationVersion: 0 -> FUSED (same bits as the reference's version 0),
+ * 1 -> DIRECTIONAL, 2 -> ATOMIC.  Every schedule but FUSED is FP64 only. */
 int e2d_set_schedule(e2d_run *run, int schedule);
+int e2d_get_schedule(e2d_run *run, int *schedule);
 /* rows marched by one wavefront of the fused step kernel (1..4096); the
  * wavefront covers 60 output columns.  Results do not depend on it. */
 int e2d_set_tile(e2d_run *run, int seg_rows);

@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from euler2d_kokkos_amd import (E2D_LAYOUT_INTERIOR, E2D_LAYOUT_LEFT, E2D_SCHEDULE_FUSED,
+from euler2d_kokkos_amd import (E2D_LAYOUT_INTERIOR, E2D_LAYOUT_LEFT, E2D_SCHEDULE_ATOMIC,
+                                E2D_SCHEDULE_DIRECTIONAL, E2D_SCHEDULE_FUSED,
                                 E2D_SCHEDULE_PASSES, E2D_U, E2D_U2, HydroParams, HydroRun)
 
 pytestmark = pytest.mark.gpu
@@ -85,8 +86,9 @@ def _compare(run, ora, tag):
         np.testing.assert_array_equal(run.get_state(which), ora.get(which), err_msg=tag)
 
 
-@pytest.mark.parametrize("schedule", [E2D_SCHEDULE_FUSED, E2D_SCHEDULE_PASSES],
-                         ids=["fused", "passes"])
+@pytest.mark.parametrize("schedule", [E2D_SCHEDULE_FUSED, E2D_SCHEDULE_PASSES,
+                                      E2D_SCHEDULE_DIRECTIONAL],
+                         ids=["fused", "passes", "directional"])
 @pytest.mark.parametrize("case", list(CASES))
 def test_loop_matches_oracle(gpu_available, case, schedule):
     p = _params(case)
@@ -166,3 +168,34 @@ def test_autotune_restores_state(gpu_available):
     run.run()
     ora.run()
     _compare(run, ora, "autotuned")
+
+
+@pytest.mark.parametrize("case", ["implode64", "blast48x72", "four_quadrant64_bc2",
+                                  "periodic_blast", "four_quadrant_hll"])
+def test_atomic_schedule_l1(gpu_available, case):
+    """Implementation version 2 adds every face flux to both neighbours with
+    FP64 atomics: the order of the additions, and so the last bits, are not
+    fixed.  Gate: relative L1 <= 1e-12 per conserved variable."""
+    p = _params(case)
+    ora = O.OracleRun(p)
+    ora.run()
+    run = _gpu_run(p, E2D_SCHEDULE_ATOMIC)
+    assert run.nStep == ora.nStep
+    assert abs(run.t - ora.t) <= 1e-12 * abs(ora.t)
+    err = O.rel_l1(O.interior(run.get_state(run.current())), O.interior(ora.current()))
+    print(f"{case}/atomic relL1={err}")
+    assert np.all(err <= RTOL_L1), err
+
+
+@pytest.mark.parametrize("version,schedule", [(0, E2D_SCHEDULE_FUSED),
+                                              (1, E2D_SCHEDULE_DIRECTIONAL),
+                                              (2, E2D_SCHEDULE_ATOMIC)])
+def test_implementation_version_selects_schedule(gpu_available, version, schedule):
+    p = O.default_params(nx=32, ny=24, nStepmax=5, tEnd=1e9, implementationVersion=version)
+    run = HydroRun(HydroParams(p.copy()))
+    assert run.get_schedule() == schedule
+    run.run()
+    ora = O.OracleRun(p)
+    ora.run()
+    err = O.rel_l1(O.interior(run.get_state(run.current())), O.interior(ora.current()))
+    assert np.all(err <= RTOL_L1), err
