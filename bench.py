@@ -42,7 +42,6 @@ DECKS = {  # deck, description, steps of the workload
                                                         "CFL 0.8", 3000),
     "implode": ("implode_256.ini", "implode 256x256 FP64 HLLC CFL 0.8 reflecting walls", 500),
 }
-TILES = (8, 10, 12, 14, 16, 20, 24, 32, 48, 64)
 
 
 PROFILE = ROOT / "profiles" / "step_fused_fp64_blast_1024x1536.md"
@@ -144,7 +143,7 @@ def split_comparison(params, run64, steps, cells, fp64_ms) -> dict:
     from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_U, HydroRun
 
     run = HydroRun(params, precision=E2D_PRECISION_FP32_SPLIT)
-    run.autotune(candidates=TILES, steps=10)
+    run.autotune(steps=10)
     run.run(20)
     run.synchronize()
     run.init_problem()
@@ -198,7 +197,7 @@ def main() -> int:
     if args.tile > 0:
         run.set_tile(args.tile)
     elif args.calibrate_copy <= 0:
-        run.autotune(candidates=TILES, steps=10)
+        run.autotune(steps=10)
 
     if args.calibrate_copy > 0:
         from euler2d_kokkos_amd import E2D_U, E2D_U2

@@ -6,6 +6,7 @@
 // and the device-resident loop control block.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -830,7 +831,30 @@ int e2d_autotune(e2d_run *r, const int *candidates, int ncand, int steps, int *b
                  double *best_ms_per_step) {
   int rc = check_run(r);
   if (rc) return rc;
-  if (!candidates || ncand <= 0 || steps <= 0) return fail(E2D_ERR_ARG, "bad argument");
+  if (steps <= 0) return fail(E2D_ERR_ARG, "bad argument");
+  std::vector<int> auto_cand;
+  if (!candidates || ncand <= 0) {
+    // A wavefront alone keeps its SIMD's VALU as busy as two sharing it, so
+    // the step time follows the rows marched per SIMD: the tiles at which
+    // the wavefronts just fit k per SIMD (k = 1..4) are the natural
+    // candidates, next to a fixed set.
+    auto_cand = {8, 12, 14, 16, 20, 24, 32, 48, 64};
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, r->device) == hipSuccess) {
+      const int simds = 4 * prop.multiProcessorCount;
+      const int nstrips = (r->p.nx + e2d::kOutCols - 1) / e2d::kOutCols;
+      for (int k = 1; k <= 4; ++k) {
+        const int segs = k * simds / nstrips;
+        if (segs < 1) continue;
+        const int tile = (r->p.ny + segs - 1) / segs;
+        if (tile >= 1 && tile <= 4096 &&
+            std::find(auto_cand.begin(), auto_cand.end(), tile) == auto_cand.end())
+          auto_cand.push_back(tile);
+      }
+    }
+    candidates = auto_cand.data();
+    ncand = (int)auto_cand.size();
+  }
   for (int c = 0; c < ncand; ++c)
     if (candidates[c] < 1 || candidates[c] > 4096) return fail(E2D_ERR_ARG, "bad candidate");
   const int schedule = r->schedule;

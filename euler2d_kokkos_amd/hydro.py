@@ -198,13 +198,16 @@ class HydroRun:
         check(load_library().e2d_get_tile(self._h, C.byref(out)))
         return out.value
 
-    def autotune(self, candidates=(8, 10, 12, 14, 16, 20, 24, 32, 48, 64), steps: int = 10):
+    def autotune(self, candidates=None, steps: int = 10):
         """Times each candidate tile on the current state (restored afterwards)
-        and keeps the fastest; returns (seg_rows, ms per step)."""
-        cand = (C.c_int * len(candidates))(*candidates)
+        and keeps the fastest; returns (seg_rows, ms per step).  candidates
+        None: a fixed set plus the tiles at which the wavefronts fit k per
+        SIMD (k = 1..4)."""
+        n = 0 if candidates is None else len(candidates)
+        cand = (C.c_int * max(n, 1))(*(candidates or ()))
         best = C.c_int(0)
         ms = C.c_double(0.0)
-        check(load_library().e2d_autotune(self._h, cand, len(candidates), steps, C.byref(best),
+        check(load_library().e2d_autotune(self._h, cand if n else None, n, steps, C.byref(best),
                                           C.byref(ms)))
         return best.value, ms.value
 

@@ -198,7 +198,9 @@ int e2d_get_schedule(e2d_run *run, int *schedule);
 int e2d_set_tile(e2d_run *run, int seg_rows);
 int e2d_get_tile(e2d_run *run, int *seg_rows);
 /* Times `steps` loop iterations for each candidate seg_rows on the current
- * state (restored afterwards, t and nStep untouched) and keeps the fastest. */
+ * state (restored afterwards, t and nStep untouched) and keeps the fastest.
+ * candidates == NULL or ncand <= 0: a fixed set plus, for k = 1..4, the
+ * smallest tile whose wavefronts fit k per SIMD of the device. */
 int e2d_autotune(e2d_run *run, const int *candidates, int ncand, int steps, int *best_seg_rows,
                  double *best_ms_per_step);
 int e2d_synchronize(e2d_run *run);

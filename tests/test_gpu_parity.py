@@ -159,6 +159,8 @@ def test_autotune_restores_state(gpu_available):
     t, n = run.t, run.nStep
     best, ms = run.autotune(candidates=(4, 9, 16), steps=3)
     assert best in (4, 9, 16) and ms > 0 and run.get_tile() == best
+    best, ms = run.autotune(steps=2)  # derived candidates
+    assert 1 <= best <= 4096 and ms > 0 and run.get_tile() == best
     assert (run.t, run.nStep) == (t, n)
     for w, b in zip((E2D_U, E2D_U2), before):
         np.testing.assert_array_equal(run.get_state(w), b)
