@@ -455,21 +455,29 @@ __global__ void boundaries_kernel(KParams kp, Grid U) {
 
 // ------------------------------------------------------------------- dt
 template <class P>
-__global__ void dt_reduce_kernel(KParams kp, Grid U, unsigned long long *invdt_bits) {
+__global__ __launch_bounds__(256) void dt_reduce_kernel(KParams kp, Grid U,
+                                                        unsigned long long *invdt_bits) {
   using T = typename P::T;
-  const long long n = (long long)kp.nx * kp.ny;
   T m = T(0.0);
-  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
-       idx += (long long)gridDim.x * blockDim.x) {
-    const int j = 2 + (int)(idx / kp.nx);
-    const int i = 2 + (int)(idx - (long long)(j - 2) * kp.nx);
-    Q4T<T> q;
-    P::prim(kp, P::load(U, (long long)j * U.pitch + i), q);
-    m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, P::rate(kp)));
+  // block b: interior rows 2+b, 2+b+G, ...; threads stride along the row
+  for (int j = 2 + blockIdx.x; j <= kp.ny + 1; j += gridDim.x)
+    for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x) {
+      Q4T<T> q;
+      P::prim(kp, P::load(U, (long long)j * U.pitch + i), q);
+      m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, P::rate(kp)));
+    }
+  // block max, then one device-scope atomic per block (the max is exact and
+  // order independent)
+  __shared__ double wm[256 / kLanes];
+  const double md = (double)wave_max(m);
+  if ((threadIdx.x & (kLanes - 1)) == 0) wm[threadIdx.x / kLanes] = md;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kLanes); w++)
+      if (wm[w] == wm[w] && wm[w] > b) b = wm[w];
+    if (b > 0.0) atomicMax(invdt_bits, dbits(b));
   }
-  m = wave_max(m);
-  const double md = (double)m;
-  if ((threadIdx.x & 63) == 0 && md == md && md > 0.0) atomicMax(invdt_bits, dbits(md));
 }
 
 // Starts a run loop: dt of the pending step from the reduced invDt, with the
@@ -909,15 +917,35 @@ __device__ __forceinline__ void advance_ctl(const KParams &kp, StepCtl *ctl) {
 
 // Folds the wave max into *invdt and reports whether this block is the last
 // of the grid to finish (all blocks' atomics are then visible).
+// The wave maxima of a block are merged per target (in the batched kernel the
+// waves of one block can belong to different instances), so that a block
+// issues one device-scope atomic per distinct target instead of one per wave.
 template <class T>
 __device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long long *invdt,
                                                        unsigned int *blocks_done) {
-  const double wm = (double)wave_max(wmax);
-  if ((threadIdx.x & (kLanes - 1)) == 0 && wm == wm && wm > 0.0 && invdt)
-    atomicMax(invdt, dbits(wm));
+  constexpr int kWaves = kBlock / kLanes;
+  __shared__ double wm[kWaves];
+  __shared__ unsigned long long *wp[kWaves];
   __shared__ bool last;
+  const double m = (double)wave_max(wmax);
+  if ((threadIdx.x & (kLanes - 1)) == 0) {
+    const int w = threadIdx.x / kLanes;
+    wm[w] = m;
+    wp[w] = invdt;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
+    for (int w = 0; w < kWaves; w++) {
+      unsigned long long *p = wp[w];
+      if (!p) continue;
+      double b = 0.0;
+      for (int v = w; v < kWaves; v++)
+        if (wp[v] == p) {
+          if (wm[v] == wm[v] && wm[v] > b) b = wm[v];
+          wp[v] = nullptr;
+        }
+      if (b > 0.0) atomicMax(p, dbits(b));
+    }
     __threadfence();
     const unsigned int prev = atomicAdd(blocks_done, 1u);
     last = (prev == gridDim.x - 1);
@@ -1228,9 +1256,7 @@ hipError_t launch_boundaries(const KParams &kp, Grid U, hipStream_t st) {
 
 hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt_bits,
                             hipStream_t st) {
-  const long long n = (long long)kp.nx * kp.ny;
-  long long blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
+  const int blocks = kp.ny < 2048 ? kp.ny : 2048;
   E2D_DISPATCH_P(kp, dt_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U,
                  invdt_bits);
   return hipGetLastError();
