@@ -946,10 +946,12 @@ __device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long lon
         }
       if (b > 0.0) atomicMax(p, dbits(b));
     }
-    __threadfence();
+    // release before counting the block done, acquire only in the last block:
+    // a full fence here would also invalidate the XCD's L2 in every block
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned int prev = atomicAdd(blocks_done, 1u);
     last = (prev == gridDim.x - 1);
-    if (last) __threadfence();
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
   return last;
