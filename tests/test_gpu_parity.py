@@ -59,6 +59,9 @@ CASES = {
     # smallc^2 below 2^-759: the wave speeds take the full square root
     "blast_tiny_smallc": dict(nx=40, ny=48, problemType=1, cfl=0.7, nStepmax=30, tEnd=1e9,
                               smallc=1e-200, blast_radius=0.15),
+    # smallr * smallp below 2^-1000: the sound speed of the CFL rate keeps the full root
+    "implode_tiny_smallr": dict(nx=36, ny=44, problemType=0, cfl=0.8, nStepmax=30, tEnd=1e9,
+                                smallr=1e-300),
 }
 
 
