@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -591,6 +592,12 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
       e2d_destroy(r);
       return fail(rc2, msg);
     }
+    // 32-bit row offsets in the step kernels need planes of at most 4 GiB;
+    // E2D_WIDE_ADDRESSING=1 selects the 64-bit form for any size
+    const char *wide = std::getenv("E2D_WIDE_ADDRESSING");
+    r->kp.wide_addr = (wide && *wide && std::strcmp(wide, "0") != 0) ||
+                      (unsigned long long)r->U.plane * e2d::grid_scalar_bytes(precision) >
+                          (1ull << 32);
     // defined contents everywhere (ghosts included) before the first init
     const size_t bytes =
       (size_t)r->U.plane * e2d::grid_planes(precision) * e2d::grid_scalar_bytes(precision);
@@ -981,6 +988,7 @@ int batch_upload_table(e2d_batch *b) {
 int batch_launch(e2d_batch *b) {
   e2d::KParams k0 = b->runs[0]->kp;
   k0.s.floor_sqrt = b->floor_sqrt;
+  for (const e2d_run *r : b->runs) k0.wide_addr |= r->kp.wide_addr;
   HIP_TRY(e2d::launch_step_batch(k0, b->insts, (int)b->runs.size(), b->ctls, b->nwaves,
                                  b->seg_rows, b->blocks_done, b->stream));
   return E2D_OK;

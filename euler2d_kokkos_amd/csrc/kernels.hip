@@ -60,6 +60,30 @@ __device__ __forceinline__ unsigned pin_offset(unsigned cb) {
   return cb;
 }
 
+// Element pointers of one row of a grid.  The 64-bit form (O32 = false) puts
+// the row's address in scalar registers and works for any grid size.  The O32
+// form adds the row's byte offset to the lane's 32-bit offset (one vector add
+// per row) and keeps the plane bases loop-invariant, which removes the 64-bit
+// scalar address arithmetic of every row; it requires every plane to be at
+// most 4 GiB, which the launchers check.
+template <class S, bool O32>
+struct RowPtr {
+  unsigned long long ra;
+  unsigned off;
+  __device__ __forceinline__ RowPtr(const Grid &g, int row, unsigned cb) {
+    if constexpr (O32) {
+      ra = reinterpret_cast<unsigned long long>(g.p);
+      off = cb + (unsigned)row * ((unsigned)g.pitch * (unsigned)sizeof(S));
+    } else {
+      ra = row_addr<S>(g, row);
+      off = cb;
+    }
+  }
+  __device__ __forceinline__ global_ptr<S> at(const Grid &g, int plane) const {
+    return plane_elem<S>(ra, g, plane, off);
+  }
+};
+
 // FP64: four double planes; the state is the value itself.
 struct P64 {
   using T = double;
@@ -76,16 +100,18 @@ struct P64 {
     double *p = static_cast<double *>(g.p);
     for (int v = 0; v < 4; v++) p[v * g.plane + off] = c.u[v];
   }
+  template <bool O32 = false>
   static __device__ __forceinline__ Cell load_rc(const Grid &g, int row, unsigned cb) {
-    const unsigned long long ra = row_addr<double>(g, row);
+    const RowPtr<double, O32> rp(g, row, cb);
     Cell c;
-    for (int v = 0; v < 4; v++) c.u[v] = *plane_elem<double>(ra, g, v, cb);
+    for (int v = 0; v < 4; v++) c.u[v] = *rp.at(g, v);
     return c;
   }
+  template <bool O32 = false>
   static __device__ __forceinline__ void store_rc(const Grid &g, int row, unsigned cb,
                                                   const Cell &c) {
-    const unsigned long long ra = row_addr<double>(g, row);
-    for (int v = 0; v < 4; v++) *plane_elem<double>(ra, g, v, cb) = c.u[v];
+    const RowPtr<double, O32> rp(g, row, cb);
+    for (int v = 0; v < 4; v++) *rp.at(g, v) = c.u[v];
   }
   static __device__ __forceinline__ void negate(Cell &c, int v) { c.u[v] = -c.u[v]; }
   static __device__ __forceinline__ Cell from_double(double d, double e, double mx, double my) {
@@ -147,21 +173,23 @@ struct PSplit {
       p[(2 * v + 1) * g.plane + off] = c.lo[v];
     }
   }
+  template <bool O32 = false>
   static __device__ __forceinline__ Cell load_rc(const Grid &g, int row, unsigned cb) {
-    const unsigned long long ra = row_addr<float>(g, row);
+    const RowPtr<float, O32> rp(g, row, cb);
     Cell c;
     for (int v = 0; v < 4; v++) {
-      c.hi[v] = *plane_elem<float>(ra, g, 2 * v, cb);
-      c.lo[v] = *plane_elem<float>(ra, g, 2 * v + 1, cb);
+      c.hi[v] = *rp.at(g, 2 * v);
+      c.lo[v] = *rp.at(g, 2 * v + 1);
     }
     return c;
   }
+  template <bool O32 = false>
   static __device__ __forceinline__ void store_rc(const Grid &g, int row, unsigned cb,
                                                   const Cell &c) {
-    const unsigned long long ra = row_addr<float>(g, row);
+    const RowPtr<float, O32> rp(g, row, cb);
     for (int v = 0; v < 4; v++) {
-      *plane_elem<float>(ra, g, 2 * v, cb) = c.hi[v];
-      *plane_elem<float>(ra, g, 2 * v + 1, cb) = c.lo[v];
+      *rp.at(g, 2 * v) = c.hi[v];
+      *rp.at(g, 2 * v + 1) = c.lo[v];
     }
   }
   static __device__ __forceinline__ void negate(Cell &c, int v) {
@@ -233,16 +261,18 @@ struct P32 {
     float *p = static_cast<float *>(g.p);
     for (int v = 0; v < 4; v++) p[v * g.plane + off] = c.u[v];
   }
+  template <bool O32 = false>
   static __device__ __forceinline__ Cell load_rc(const Grid &g, int row, unsigned cb) {
-    const unsigned long long ra = row_addr<float>(g, row);
+    const RowPtr<float, O32> rp(g, row, cb);
     Cell c;
-    for (int v = 0; v < 4; v++) c.u[v] = *plane_elem<float>(ra, g, v, cb);
+    for (int v = 0; v < 4; v++) c.u[v] = *rp.at(g, v);
     return c;
   }
+  template <bool O32 = false>
   static __device__ __forceinline__ void store_rc(const Grid &g, int row, unsigned cb,
                                                   const Cell &c) {
-    const unsigned long long ra = row_addr<float>(g, row);
-    for (int v = 0; v < 4; v++) *plane_elem<float>(ra, g, v, cb) = c.u[v];
+    const RowPtr<float, O32> rp(g, row, cb);
+    for (int v = 0; v < 4; v++) *rp.at(g, v) = c.u[v];
   }
   static __device__ __forceinline__ void negate(Cell &c, int v) { c.u[v] = -c.u[v]; }
   static __device__ __forceinline__ Cell from_double(double d, double e, double mx, double my) {
@@ -739,7 +769,7 @@ struct MarchState {
 // One iteration over loaded row r.  STAGES is a compile-time set, so the
 // steady-state iteration is one branch-free basic block in which the compiler
 // can interleave the independent x-face and y-face Riemann chains.
-template <int RS, class P, int STAGES, bool PREFETCH, bool RATE>
+template <int RS, class P, int STAGES, bool PREFETCH, bool RATE, bool O32>
 __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, const Grid &out,
                                           MarchState<P> &m, int r, int c0, bool flipx,
                                           typename P::T dtdx, typename P::T dtdy,
@@ -753,7 +783,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     bool flipy;
     const int rc = min(r + 1, kp.ny + 3);
     const int r0 = ghost_map_uniform(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
-    m.un = P::load_rc(in, r0, pin_offset((unsigned)c0 * sizeof(T)));
+    m.un = P::template load_rc<O32>(in, r0, pin_offset((unsigned)c0 * sizeof(T)));
     if (flipx) P::negate(m.un, IU);
     if (flipy) P::negate(m.un, IV);
   }
@@ -801,13 +831,13 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
   if ((ghost_col || row_ghost) && r >= row_lo && r <= row_hi) {
     const unsigned cb = pin_offset((unsigned)col * sizeof(T));
-    P::store_rc(in, r, cb, uc);
-    P::store_rc(out, r, cb, uc);
+    P::template store_rc<O32>(in, r, cb, uc);
+    P::template store_rc<O32>(out, r, cb, uc);
   }
   if constexpr ((STAGES & kUpdate) != 0) {
     const int b = r - 2;
     if (out_col && b <= kp.ny + 1) {
-      P::store_rc(out, b, pin_offset((unsigned)col * sizeof(T)), unew);
+      P::template store_rc<O32>(out, b, pin_offset((unsigned)col * sizeof(T)), unew);
       if constexpr (RATE) {  // next-step CFL rate of the new state
         Q4T<T> qn;
         P::prim(kp, unew, qn);
@@ -833,7 +863,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
 // slopes of row j_base-1; k = 3 the first y and x faces; k = 4 .. seg_rows+2
 // is the steady state (all stages, branch-free); k = seg_rows+3 finishes the
 // last y face and update.
-template <int RS, class P, bool RATE>
+template <int RS, class P, bool RATE, bool O32>
 __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in, const Grid &out,
                                                double dt, int seg_rows, int nstrips, int strip,
                                                int seg) {
@@ -868,7 +898,7 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
     bool flipy;
     const int r0 =
         ghost_map_uniform(min(j_base - 2, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
-    m.un = P::load_rc(in, r0, (unsigned)c0 * sizeof(T));
+    m.un = P::template load_rc<O32>(in, r0, (unsigned)c0 * sizeof(T));
     if (flipx) P::negate(m.un, IU);
     if (flipy) P::negate(m.un, IV);
   }
@@ -879,8 +909,8 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
 
   const int r0 = j_base - 2;
 #define E2D_ROW(ST, PF, RT, k)                                                               \
-  march_row<RS, P, ST, PF, RT>(kp, in, out, m, r0 + (k), c0, flipx, dtdx, dtdy, ghost_col,   \
-                               out_col, col, wlo, whi)
+  march_row<RS, P, ST, PF, RT, O32>(kp, in, out, m, r0 + (k), c0, flipx, dtdx, dtdy,          \
+                                    ghost_col, out_col, col, wlo, whi)
   E2D_ROW(0, true, false, 0);
   E2D_ROW(0, true, false, 1);
   E2D_ROW(kSlopes, true, false, 2);
@@ -957,8 +987,9 @@ __device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long lon
   return last;
 }
 
-// LOOP = false: explicit dt (HydroRun::godunov_unsplit); true: run-loop mode
-template <int RS, class P, bool LOOP>
+// LOOP = false: explicit dt (HydroRun::godunov_unsplit); true: run-loop mode.
+// O32: 32-bit row offsets (every plane at most 4 GiB).
+template <int RS, class P, bool LOOP, bool O32>
 __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in, Grid out,
                                                             double dt_arg, StepCtl *ctl,
                                                             int seg_rows, int nstrips, int nwaves) {
@@ -970,8 +1001,8 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
       __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6));
   T wmax = T(0.0);
   if (wave < nwaves)
-    wmax = march<RS, P, LOOP>(kp, in, out, LOOP ? ctl->dt : dt_arg, seg_rows, nstrips,
-                              wave % nstrips, wave / nstrips);
+    wmax = march<RS, P, LOOP, O32>(kp, in, out, LOOP ? ctl->dt : dt_arg, seg_rows, nstrips,
+                                   wave % nstrips, wave / nstrips);
   if (!LOOP) return;
   // run-loop epilogue: next-step invDt (exact max, order independent) and the
   // last block advances the loop state
@@ -988,7 +1019,7 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
 // (binary search over wave_begin), on that instance's strip / segment; its
 // in/out arrays follow the instance's own step parity.  The last block of the
 // grid advances every instance's loop state.
-template <int RS, class P>
+template <int RS, class P, bool O32>
 __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__restrict__ insts,
                                                             int ninst,
                                                             StepCtl *ctls, int nwaves,
@@ -1019,8 +1050,8 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__r
       const KParams kp = bi.kp;
       const Grid gin = even ? bi.U : bi.U2;
       const Grid gout = even ? bi.U2 : bi.U;
-      wmax = march<RS, P, true>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips, w % bi.nstrips,
-                                w / bi.nstrips);
+      wmax = march<RS, P, true, O32>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips,
+                                     w % bi.nstrips, w / bi.nstrips);
     }
   }
   const bool last = reduce_rate_last_block(wmax, inst >= 0 ? &ctls[inst].invdt : nullptr,
@@ -1128,6 +1159,13 @@ __global__ void copy_kernel(const double2 *__restrict__ src, double2 *__restrict
     dst[k] = src[k];
 }
 
+// 32-bit row offsets reach every element of a plane of at most 4 GiB
+bool off32_ok(const KParams &kp, const Grid &g) {
+  return !kp.wide_addr && (unsigned long long)g.plane *
+                                  (unsigned long long)(kp.prec == E2D_PRECISION_FP64 ? 8 : 4) <=
+                              (1ull << 32);
+}
+
 dim3 grid2d(int nx, int ny, dim3 blk) {
   return dim3((unsigned)((nx + blk.x - 1) / blk.x), (unsigned)((ny + blk.y - 1) / blk.y));
 }
@@ -1140,12 +1178,15 @@ void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
   const int nwaves = nstrips * nsegs;
   const int waves_per_block = kBlock / kLanes;
   const int blocks = (nwaves + waves_per_block - 1) / waves_per_block;
-  if (ctl)
-    hipLaunchKernelGGL((step_fused_kernel<RS, P, true>), dim3(blocks), dim3(kBlock), 0, st, kp, in,
-                       out, dt, ctl, seg_rows, nstrips, nwaves);
+  if (ctl && off32_ok(kp, in) && off32_ok(kp, out))
+    hipLaunchKernelGGL((step_fused_kernel<RS, P, true, true>), dim3(blocks), dim3(kBlock), 0, st,
+                       kp, in, out, dt, ctl, seg_rows, nstrips, nwaves);
+  else if (ctl)
+    hipLaunchKernelGGL((step_fused_kernel<RS, P, true, false>), dim3(blocks), dim3(kBlock), 0, st,
+                       kp, in, out, dt, ctl, seg_rows, nstrips, nwaves);
   else
-    hipLaunchKernelGGL((step_fused_kernel<RS, P, false>), dim3(blocks), dim3(kBlock), 0, st, kp,
-                       in, out, dt, ctl, seg_rows, nstrips, nwaves);
+    hipLaunchKernelGGL((step_fused_kernel<RS, P, false, false>), dim3(blocks), dim3(kBlock), 0,
+                       st, kp, in, out, dt, ctl, seg_rows, nstrips, nwaves);
 }
 
 template <class P>
@@ -1364,9 +1405,15 @@ hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int
                              StepCtl *ctls_dev, int nwaves, int seg_rows,
                              unsigned int *blocks_done, hipStream_t st) {
   const int blocks = (nwaves + kBlock / kLanes - 1) / (kBlock / kLanes);
-#define E2D_BATCH_LAUNCH(RS, P)                                                              \
-  hipLaunchKernelGGL((step_batch_kernel<RS, P>), dim3(blocks), dim3(kBlock), 0, st, insts_dev, \
-                     ninst, ctls_dev, nwaves, seg_rows, blocks_done)
+#define E2D_BATCH_LAUNCH(RS, P)                                                            \
+  do {                                                                                     \
+    if (kp0.wide_addr)                                                                     \
+      hipLaunchKernelGGL((step_batch_kernel<RS, P, false>), dim3(blocks), dim3(kBlock), 0,   \
+                         st, insts_dev, ninst, ctls_dev, nwaves, seg_rows, blocks_done);   \
+    else                                                                                   \
+      hipLaunchKernelGGL((step_batch_kernel<RS, P, true>), dim3(blocks), dim3(kBlock), 0,    \
+                         st, insts_dev, ninst, ctls_dev, nwaves, seg_rows, blocks_done);   \
+  } while (0)
   if (kp0.prec == E2D_PRECISION_FP32_SPLIT) {
     if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, PSplit);
     else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, PSplit);

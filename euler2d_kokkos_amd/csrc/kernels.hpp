@@ -36,6 +36,10 @@ struct KParams {
   // split path: primitive conversion redone in FP64 where the internal energy
   // is below this fraction of the total energy (p < kin_frac * (gamma-1) * E)
   float kin_frac;
+  // 1: the run-loop and batched step kernels address rows with 64-bit scalar
+  // arithmetic even where every plane is at most 4 GiB (set for planes above
+  // 4 GiB, or through E2D_WIDE_ADDRESSING=1 to exercise that form)
+  int wide_addr;
 };
 
 // Run-loop control block kept in device memory (time loop of main.cpp:100-145
@@ -106,7 +110,8 @@ int sums_partials_needed(const KParams &kp);
 // wavefronts the fused step launches for one grid (and its strip count)
 int batch_waves(const KParams &kp, int seg_rows, int *nstrips);
 // one step of every active instance; kp0 gives the precision and solver of all
-// (kp0.s.floor_sqrt must hold for every instance)
+// (kp0.s.floor_sqrt must hold for every instance; kp0.wide_addr must be set
+// when any instance has a plane above 4 GiB)
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
                              StepCtl *ctls_dev, int nwaves, int seg_rows,
                              unsigned int *blocks_done, hipStream_t st);

@@ -47,6 +47,25 @@ def test_batch_equals_single_runs(gpu_available, precision, use_graph):
                                       err_msg=f"instance {i}")
 
 
+@pytest.mark.parametrize("precision", [E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT],
+                         ids=["fp64", "split"])
+def test_batch_wide_addressing(gpu_available, precision, monkeypatch):
+    """A batch with the 64-bit row addressing (forced) equals single runs with
+    the 32-bit one."""
+    ps = mixed_instances()
+    monkeypatch.setenv("E2D_WIDE_ADDRESSING", "1")
+    batch = HydroBatch(ps, precision=precision)
+    monkeypatch.delenv("E2D_WIDE_ADDRESSING")
+    batch.set_tile(16)
+    batch.run(64)
+    for i, p in enumerate(ps):
+        run = HydroRun(HydroParams(p.copy()), precision=precision)
+        run.run()
+        assert batch.status(i) == (run.t, run.nStep), i
+        np.testing.assert_array_equal(batch.state(i), run.get_state(run.current()),
+                                      err_msg=f"instance {i}")
+
+
 def test_batch_fp64_equals_oracle(gpu_available):
     ps = mixed_instances()[:3]
     batch = HydroBatch(ps, precision=E2D_PRECISION_FP64)

@@ -101,6 +101,39 @@ def test_loop_matches_oracle(gpu_available, case, schedule):
     _compare(run, ora, f"{case}/{schedule}")
 
 
+@pytest.mark.parametrize("case", list(CASES))
+def test_loop_wide_addressing_matches_oracle(gpu_available, case, monkeypatch):
+    """The run loop with the 64-bit row addressing that the step kernels use
+    for planes above 4 GiB (forced here on small grids)."""
+    monkeypatch.setenv("E2D_WIDE_ADDRESSING", "1")
+    p = _params(case)
+    ora = O.OracleRun(p)
+    ora.run()
+    run = _gpu_run(p, E2D_SCHEDULE_FUSED)
+    _compare(run, ora, f"{case}/wide")
+
+
+def test_planes_above_4gib(gpu_available):
+    """1000 x 540000 cells: each FP64 plane holds 4.35 GB, so the step kernel
+    addresses rows with 64-bit offsets; the blast sits at rows ~539000, beyond
+    the 4 GiB offset.  Two steps of the fused loop must equal two steps of the
+    reference pass schedule (equal time and equal conserved sums, which the
+    same deterministic reduction computes from both arrays)."""
+    p = O.default_params(nx=1000, ny=540000, ymax=540.0, problemType=1, cfl=0.7,
+                         nStepmax=2, tEnd=1e9, blast_radius=0.05, blast_center_x=0.5,
+                         blast_center_y=539.0)
+    res = []
+    for schedule in (E2D_SCHEDULE_FUSED, E2D_SCHEDULE_PASSES):
+        run = HydroRun(HydroParams(p.copy()), schedule=schedule)
+        run.run()
+        res.append((run.nStep, run.t, run.conserved_sums(run.current()).tolist()))
+        print(schedule, res[-1], flush=True)
+        run.close()
+        del run
+    assert res[0][0] == 2
+    assert res[0] == res[1]
+
+
 @pytest.mark.parametrize("case", ["implode64", "blast48x72"])
 def test_reference_api_steps(gpu_available, case):
     """compute_dt / godunov_unsplit / make_boundaries called step by step."""

@@ -142,3 +142,19 @@ def test_plain_fp32_comparator(gpu_available):
     assert l1_fp32[0] < 1e-2 and l1_split[0] <= L1_TOL
     assert np.all(out["split"][1] <= 1e-12)
     assert np.all(out["fp32"][1] > out["split"][1])
+
+
+@pytest.mark.parametrize("precision", [E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64],
+                         ids=["split", "fp64"])
+def test_wide_addressing_same_bits(gpu_available, precision, monkeypatch):
+    """The 64-bit row addressing (planes above 4 GiB; forced here) and the
+    32-bit one give the same arrays bit for bit."""
+    p = O.default_params(**CASES["shocked_bubble"])
+    a = HydroRun(HydroParams(p.copy()), precision=precision)
+    a.run(60)
+    monkeypatch.setenv("E2D_WIDE_ADDRESSING", "1")
+    b = HydroRun(HydroParams(p.copy()), precision=precision)
+    b.run(60)
+    assert (a.t, a.nStep) == (b.t, b.nStep)
+    for w in (E2D_U, E2D_U2):
+        np.testing.assert_array_equal(a.get_state(w), b.get_state(w))
