@@ -140,18 +140,13 @@ struct P64 {
   }
 };
 
-// Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no contraction).
-__device__ __forceinline__ float two_sum(float a, float b, float &e) {
-  const float s = a + b;
-  const float bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-  return s;
-}
-
-// Split precision: hi and lo FP32 planes per variable.  Fluxes are FP32; each
-// one is added to the state with TwoSum so that the exact sum hi + lo changes
-// by exactly the flux (the same FP32 flux value leaves one cell and enters its
-// neighbour), which keeps the global sums conserved to FP32^2 rounding.
+// Split precision: hi and lo FP32 planes per variable, about 48 significant
+// bits of state.  Fluxes are FP32.  The update adds the four face fluxes to
+// hi + lo in FP64, in the reference's order, then stores the FP32 rounding of
+// the sum as hi and the FP32 rounding of the remainder (exact in FP64) as lo.
+// The same FP32 flux value leaves one cell and enters its neighbour, so the
+// global sums move only by the roundings of lo, about 2^-48 of a cell's value
+// per update.
 struct PSplit {
   using T = float;
   struct Cell {
@@ -228,16 +223,14 @@ struct PSplit {
                                                 const Q4T<T> &fyt) {
     Cell o;
     for (int v = 0; v < 4; v++) {
-      float e0, e1, e2, e3;
-      float s = two_sum(c.hi[v], fxl.v[v], e0);
-      s = two_sum(s, -fxr.v[v], e1);
-      s = two_sum(s, fyb.v[v], e2);
-      s = two_sum(s, -fyt.v[v], e3);
-      const float lo = c.lo[v] + (((e0 + e1) + e2) + e3);
-      // renormalise (Fast2Sum, |s| >= |lo|)
-      const float hi = s + lo;
+      double x = (double)c.hi[v] + (double)c.lo[v];
+      x += (double)fxl.v[v];
+      x -= (double)fxr.v[v];
+      x += (double)fyb.v[v];
+      x -= (double)fyt.v[v];
+      const float hi = (float)x;
       o.hi[v] = hi;
-      o.lo[v] = lo - (hi - s);
+      o.lo[v] = (float)(x - (double)hi);
     }
     return o;
   }
