@@ -70,6 +70,8 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--split-tile", type=int, default=0,
+                    help="tile of the split-precision run (default: autotuned)")
     ap.add_argument("--no-split", action="store_true",
                     help="skip the FP32 split-precision comparison run")
     ap.add_argument("--tile", type=int, default=0,
@@ -133,7 +135,7 @@ def cpu_baseline(deck: Path, seconds: float) -> dict:
                       f"(FP64) with OpenMP over {threads} threads, {el:.1f} s"}
 
 
-def split_comparison(params, run64, steps, cells, fp64_ms) -> dict:
+def split_comparison(params, run64, steps, cells, fp64_ms, tile=0) -> dict:
     """Same workload in FP32 split precision: throughput, L1 error against the
     FP64 run's final state, FP64-summed drift of mass and energy over the run
     (nonzero where the boundaries let mass through), and the distance of the
@@ -143,7 +145,10 @@ def split_comparison(params, run64, steps, cells, fp64_ms) -> dict:
     from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_U, HydroRun
 
     run = HydroRun(params, precision=E2D_PRECISION_FP32_SPLIT)
-    run.autotune(steps=10)
+    if tile > 0:
+        run.set_tile(tile)
+    else:
+        run.autotune(steps=10)
     run.run(20)
     run.synchronize()
     run.init_problem()
@@ -251,7 +256,8 @@ def main() -> int:
         "final_t": run.t,
     }
     if not args.no_split:
-        line["fp32_split"] = split_comparison(params, run, args.steps, cells, per_step_ms)
+        line["fp32_split"] = split_comparison(params, run, args.steps, cells, per_step_ms,
+                                              args.split_tile)
     if not args.no_batch:
         line["batched"] = batch_line(args.batch_steps)
     if not args.no_cpu:
