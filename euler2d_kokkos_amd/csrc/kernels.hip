@@ -762,7 +762,7 @@ struct MarchState {
 // One iteration over loaded row r.  STAGES is a compile-time set, so the
 // steady-state iteration is one branch-free basic block in which the compiler
 // can interleave the independent x-face and y-face Riemann chains.
-template <int RS, class P, int STAGES, bool PREFETCH, bool RATE, bool O32>
+template <int RS, class P, int STAGES, bool PREFETCH, bool RATE, bool O32, bool EDGE>
 __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, const Grid &out,
                                           MarchState<P> &m, int r, int c0, bool flipx,
                                           typename P::T dtdx, typename P::T dtdy,
@@ -772,13 +772,15 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   using Cell = typename P::Cell;
   const SchemeT<T> &s = P::scheme(kp);
   const Cell uc = m.un;
-  if constexpr (PREFETCH) {
+  if constexpr (PREFETCH && EDGE) {
     bool flipy;
     const int rc = min(r + 1, kp.ny + 3);
     const int r0 = ghost_map_uniform(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
     m.un = P::template load_rc<O32>(in, r0, pin_offset((unsigned)c0 * sizeof(T)));
     if (flipx) P::negate(m.un, IU);
     if (flipy) P::negate(m.un, IV);
+  } else if constexpr (PREFETCH) {  // interior wavefront: row r + 1 is inside the domain
+    m.un = P::template load_rc<O32>(in, r + 1, pin_offset((unsigned)c0 * sizeof(T)));
   }
   Q4T<T> q0;
   P::prim(kp, uc, q0);
@@ -821,15 +823,17 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
 
   // stores last, so that the arithmetic above stays in one basic block
   // ghost cells of row r owned by this wave: materialise them in both arrays
-  const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
-  if ((ghost_col || row_ghost) && r >= row_lo && r <= row_hi) {
-    const unsigned cb = pin_offset((unsigned)col * sizeof(T));
-    P::template store_rc<O32>(in, r, cb, uc);
-    P::template store_rc<O32>(out, r, cb, uc);
+  if constexpr (EDGE) {
+    const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
+    if ((ghost_col || row_ghost) && r >= row_lo && r <= row_hi) {
+      const unsigned cb = pin_offset((unsigned)col * sizeof(T));
+      P::template store_rc<O32>(in, r, cb, uc);
+      P::template store_rc<O32>(out, r, cb, uc);
+    }
   }
   if constexpr ((STAGES & kUpdate) != 0) {
     const int b = r - 2;
-    if (out_col && b <= kp.ny + 1) {
+    if (out_col && (!EDGE || b <= kp.ny + 1)) {
       P::template store_rc<O32>(out, b, pin_offset((unsigned)col * sizeof(T)), unew);
       if constexpr (RATE) {  // next-step CFL rate of the new state
         Q4T<T> qn;
@@ -856,7 +860,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
 // slopes of row j_base-1; k = 3 the first y and x faces; k = 4 .. seg_rows+2
 // is the steady state (all stages, branch-free); k = seg_rows+3 finishes the
 // last y face and update.
-template <int RS, class P, bool RATE, bool O32>
+template <int RS, class P, bool RATE, bool O32, bool EDGE>
 __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in, const Grid &out,
                                                double dt, int seg_rows, int nstrips, int strip,
                                                int seg) {
@@ -870,30 +874,40 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
   const int col = i_base - 2 + lane;
   const int j_base = 2 + seg_rows * seg;
 
-  bool flipx;
-  const int colc = min(col, kp.nx + 3);
-  const int c0 = ghost_map(colc, kp.nx, kp.bcxmin, kp.bcxmax, flipx);
-  const bool col_ghost = col < 2 || (col > kp.nx + 1 && col <= kp.nx + 3);
-  const bool col_owned = (lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1) ||
-                         (strip == 0 && lane < 2) ||
-                         (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
-  const bool out_col = lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1;
-  // Ghost cells this wave writes: owned column, owned row, and a ghost column or
-  // ghost row.  The owned rows of a segment are contiguous: [row_lo, row_hi].
-  const int row_lo = seg == 0 ? 0 : j_base;
-  const int row_hi = seg == nsegs - 1 ? kp.ny + 3 : min(j_base + seg_rows - 1, kp.ny + 1);
-  const bool ghost_col = col_ghost && col_owned;
-  const int wlo = col_owned ? row_lo : 1;   // empty range for columns not owned
-  const int whi = col_owned ? row_hi : -1;
+  // an interior wavefront (EDGE = false) loads only cells inside the domain,
+  // owns no ghost cell and updates valid rows only (interior_wave())
+  bool flipx = false;
+  int c0 = col;
+  bool out_col = lane >= 2 && lane < 2 + kOutCols;
+  bool ghost_col = false;
+  int wlo = 1, whi = -1;
+  if constexpr (EDGE) {
+    const int colc = min(col, kp.nx + 3);
+    c0 = ghost_map(colc, kp.nx, kp.bcxmin, kp.bcxmax, flipx);
+    const bool col_ghost = col < 2 || (col > kp.nx + 1 && col <= kp.nx + 3);
+    const bool col_owned = (lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1) ||
+                           (strip == 0 && lane < 2) ||
+                           (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
+    out_col = out_col && col <= kp.nx + 1;
+    // Ghost cells this wave writes: owned column, owned row, and a ghost column or
+    // ghost row.  The owned rows of a segment are contiguous: [row_lo, row_hi].
+    const int row_lo = seg == 0 ? 0 : j_base;
+    const int row_hi = seg == nsegs - 1 ? kp.ny + 3 : min(j_base + seg_rows - 1, kp.ny + 1);
+    ghost_col = col_ghost && col_owned;
+    wlo = col_owned ? row_lo : 1;  // empty range for columns not owned
+    whi = col_owned ? row_hi : -1;
+  }
 
   MarchState<P> m;
-  {
+  if constexpr (EDGE) {
     bool flipy;
     const int r0 =
         ghost_map_uniform(min(j_base - 2, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
     m.un = P::template load_rc<O32>(in, r0, (unsigned)c0 * sizeof(T));
     if (flipx) P::negate(m.un, IU);
     if (flipy) P::negate(m.un, IV);
+  } else {
+    m.un = P::template load_rc<O32>(in, j_base - 2, (unsigned)c0 * sizeof(T));
   }
   m.ua = m.un;
   m.ub = m.un;
@@ -902,8 +916,8 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
 
   const int r0 = j_base - 2;
 #define E2D_ROW(ST, PF, RT, k)                                                               \
-  march_row<RS, P, ST, PF, RT, O32>(kp, in, out, m, r0 + (k), c0, flipx, dtdx, dtdy,          \
-                                    ghost_col, out_col, col, wlo, whi)
+  march_row<RS, P, ST, PF, RT, O32, EDGE>(kp, in, out, m, r0 + (k), c0, flipx, dtdx, dtdy,    \
+                                          ghost_col, out_col, col, wlo, whi)
   E2D_ROW(0, true, false, 0);
   E2D_ROW(0, true, false, 1);
   E2D_ROW(kSlopes, true, false, 2);
@@ -980,6 +994,17 @@ __device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long lon
   return last;
 }
 
+// True when every row and column the wavefront of (strip, seg) loads lies
+// inside the domain: it then maps, flips and writes no ghost cell, and every
+// row it updates is an interior row.  Columns loaded: 60 strip .. 60 strip + 63;
+// rows loaded: seg_rows seg .. seg_rows (seg + 1) + 3.
+__device__ __forceinline__ bool interior_wave(const KParams &kp, int seg_rows, int strip,
+                                              int seg) {
+  const int c_lo = kOutCols * strip, r_lo = seg_rows * seg;
+  return c_lo >= 2 && c_lo + kLanes - 1 <= kp.nx + 1 && r_lo >= 2 &&
+         r_lo + seg_rows + 3 <= kp.ny + 1;
+}
+
 // LOOP = false: explicit dt (HydroRun::godunov_unsplit); true: run-loop mode.
 // O32: 32-bit row offsets (every plane at most 4 GiB).
 template <int RS, class P, bool LOOP, bool O32>
@@ -993,9 +1018,14 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
   const int wave =
       __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6));
   T wmax = T(0.0);
-  if (wave < nwaves)
-    wmax = march<RS, P, LOOP, O32>(kp, in, out, LOOP ? ctl->dt : dt_arg, seg_rows, nstrips,
-                                   wave % nstrips, wave / nstrips);
+  if (wave < nwaves) {
+    const int strip = wave % nstrips, seg = wave / nstrips;
+    const double dt = LOOP ? ctl->dt : dt_arg;
+    if (interior_wave(kp, seg_rows, strip, seg))  // uniform
+      wmax = march<RS, P, LOOP, O32, false>(kp, in, out, dt, seg_rows, nstrips, strip, seg);
+    else
+      wmax = march<RS, P, LOOP, O32, true>(kp, in, out, dt, seg_rows, nstrips, strip, seg);
+  }
   if (!LOOP) return;
   // run-loop epilogue: next-step invDt (exact max, order independent) and the
   // last block advances the loop state
@@ -1043,8 +1073,13 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__r
       const KParams kp = bi.kp;
       const Grid gin = even ? bi.U : bi.U2;
       const Grid gout = even ? bi.U2 : bi.U;
-      wmax = march<RS, P, true, O32>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips,
-                                     w % bi.nstrips, w / bi.nstrips);
+      const int strip = w % bi.nstrips, seg = w / bi.nstrips;
+      if (interior_wave(kp, seg_rows, strip, seg))
+        wmax = march<RS, P, true, O32, false>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips,
+                                              strip, seg);
+      else
+        wmax = march<RS, P, true, O32, true>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips,
+                                             strip, seg);
     }
   }
   const bool last = reduce_rate_last_block(wmax, inst >= 0 ? &ctls[inst].invdt : nullptr,

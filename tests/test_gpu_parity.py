@@ -113,6 +113,24 @@ def test_loop_wide_addressing_matches_oracle(gpu_available, case, monkeypatch):
     _compare(run, ora, f"{case}/wide")
 
 
+@pytest.mark.parametrize("tile", [3, 16, 28])
+@pytest.mark.parametrize("bc", [1, 3], ids=["reflecting", "periodic"])
+def test_interior_wavefronts_match_oracle(gpu_available, tile, bc):
+    """Grids wide and tall enough for interior wavefronts (every loaded cell
+    inside the domain: columns 60 s .. 60 s + 63 and rows tile k .. tile (k+1)
+    + 3), which run a march without ghost handling, next to edge wavefronts."""
+    p = O.default_params(nx=250, ny=181, problemType=1, cfl=0.7, nStepmax=25, tEnd=1e9,
+                         blast_radius=0.2, blast_center_x=0.45, blast_center_y=0.6,
+                         boundary_type_xmin=bc, boundary_type_xmax=bc,
+                         boundary_type_ymin=bc, boundary_type_ymax=bc)
+    ora = O.OracleRun(p)
+    ora.run()
+    run = HydroRun(HydroParams(p.copy()))
+    run.set_tile(tile)
+    run.run()
+    _compare(run, ora, f"interior/{tile}/{bc}")
+
+
 def test_planes_above_4gib(gpu_available):
     """1000 x 540000 cells: each FP64 plane holds 4.35 GB, so the step kernel
     addresses rows with 64-bit offsets; the blast sits at rows ~539000, beyond
