@@ -24,9 +24,15 @@ enum { ID = 0, IP = 1, IU = 2, IV = 3 };
 template <class T>
 struct Ops;
 
+// mad(a, b, c) = a * b + c: for FP64 the product and the sum are rounded
+// separately, exactly as the reference writes the expression (the FP64 path
+// must keep every rounding); for FP32 it is one fused multiply-add.  Callers
+// pass the operands so that the FP64 form is the reference's own expression
+// tree (x - a * b is written mad(-a, b, x): negating a factor is exact).
 template <>
 struct Ops<double> {
   static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double mad(double a, double b, double c) { return a * b + c; }
   static __device__ __forceinline__ double sqroot(double x) { return sqrt(x); }
   static __device__ __forceinline__ double mx(double a, double b) { return fmax(a, b); }
   static __device__ __forceinline__ double mn(double a, double b) { return fmin(a, b); }
@@ -59,6 +65,9 @@ template <>
 struct Ops<float> {
   static __device__ __forceinline__ float div(float a, float b) {
     return a * __builtin_amdgcn_rcpf(b);
+  }
+  static __device__ __forceinline__ float mad(float a, float b, float c) {
+    return __builtin_fmaf(a, b, c);
   }
   static __device__ __forceinline__ float sqroot(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float mx(float a, float b) { return fmaxf(a, b); }
@@ -103,7 +112,7 @@ __device__ __forceinline__ void prim_no_c(const SchemeT<T> &s, T u0, T u1, T u2,
   const T d = O::mx(u0, s.smallr);
   const T ux = O::div(u2, d);
   const T uy = O::div(u3, d);
-  const T eken = T(0.5) * (ux * ux + uy * uy);
+  const T eken = T(0.5) * O::mad(uy, uy, ux * ux);
   const T e = O::div(u1, d) - eken;
   const T p = O::mx((s.gamma0 - T(1.0)) * d * e, d * s.smallp);
   q.v[ID] = d;
@@ -192,10 +201,12 @@ __device__ __forceinline__ SourcesT<T> trace_sources(const SchemeT<T> &s, const 
   using O = Ops<T>;
   const T r = q.v[ID], p = q.v[IP], u = q.v[IU], v = q.v[IV];
   SourcesT<T> o;
-  o.sr0 = -u * dqX.v[ID] - v * dqY.v[ID] - (dqX.v[IU] + dqY.v[IV]) * r;
-  o.sp0 = -u * dqX.v[IP] - v * dqY.v[IP] - (dqX.v[IU] + dqY.v[IV]) * s.gamma0 * p;
-  o.su0 = -u * dqX.v[IU] - v * dqY.v[IU] - O::div((dqX.v[IP]), r);
-  o.sv0 = -u * dqX.v[IV] - v * dqY.v[IV] - O::div((dqY.v[IP]), r);
+  // ((-u*a - v*b) - c*d) per source term, with the products fused in FP32
+  const T divu = dqX.v[IU] + dqY.v[IV];
+  o.sr0 = O::mad(-divu, r, O::mad(-v, dqY.v[ID], -u * dqX.v[ID]));
+  o.sp0 = O::mad(-(divu * s.gamma0), p, O::mad(-v, dqY.v[IP], -u * dqX.v[IP]));
+  o.su0 = O::mad(-v, dqY.v[IU], -u * dqX.v[IU]) - O::div((dqX.v[IP]), r);
+  o.sv0 = O::mad(-v, dqY.v[IV], -u * dqX.v[IV]) - O::div((dqY.v[IP]), r);
   return o;
 }
 
@@ -205,19 +216,14 @@ __device__ __forceinline__ SourcesT<T> trace_sources(const SchemeT<T> &s, const 
 template <class T>
 __device__ __forceinline__ void trace_face(const SchemeT<T> &s, const Q4T<T> &q, const Q4T<T> &dq,
                                            const SourcesT<T> &src, T dtd, bool plus, Q4T<T> &f) {
-  const T h = T(0.5);
-  if (plus) {
-    f.v[ID] = q.v[ID] + h * dq.v[ID] + src.sr0 * dtd * h;
-    f.v[IP] = q.v[IP] + h * dq.v[IP] + src.sp0 * dtd * h;
-    f.v[IU] = q.v[IU] + h * dq.v[IU] + src.su0 * dtd * h;
-    f.v[IV] = q.v[IV] + h * dq.v[IV] + src.sv0 * dtd * h;
-  } else {
-    f.v[ID] = q.v[ID] - h * dq.v[ID] + src.sr0 * dtd * h;
-    f.v[IP] = q.v[IP] - h * dq.v[IP] + src.sp0 * dtd * h;
-    f.v[IU] = q.v[IU] - h * dq.v[IU] + src.su0 * dtd * h;
-    f.v[IV] = q.v[IV] - h * dq.v[IV] + src.sv0 * dtd * h;
-  }
-  f.v[ID] = Ops<T>::mx(s.smallr, f.v[ID]);
+  using O = Ops<T>;
+  // (q +- h*dq) + (src*dtd)*h
+  const T h = T(0.5), hs = plus ? h : -h;
+  f.v[ID] = O::mad(src.sr0 * dtd, h, O::mad(hs, dq.v[ID], q.v[ID]));
+  f.v[IP] = O::mad(src.sp0 * dtd, h, O::mad(hs, dq.v[IP], q.v[IP]));
+  f.v[IU] = O::mad(src.su0 * dtd, h, O::mad(hs, dq.v[IU], q.v[IU]));
+  f.v[IV] = O::mad(src.sv0 * dtd, h, O::mad(hs, dq.v[IV], q.v[IV]));
+  f.v[ID] = O::mx(s.smallr, f.v[ID]);
 }
 
 // riemann_hllc, HydroBaseFunctor.h:704-809.  `ln`/`tn` are the indices of the
@@ -235,16 +241,16 @@ __device__ __forceinline__ void hllc(const SchemeT<T> &s, const Q4T<T> &ql, cons
   const T pl = O::mx(ql.v[IP], rl * s.smallp);
   const T ul = ql.v[ln];
   T ecinl = h * rl * ul * ul;
-  ecinl += h * rl * ql.v[tn] * ql.v[tn];
-  const T etotl = pl * s.entho + ecinl;
+  ecinl = O::mad(h * rl * ql.v[tn], ql.v[tn], ecinl);
+  const T etotl = O::mad(pl, s.entho, ecinl);
   const T ptotl = pl;
 
   const T rr = O::mx(qr.v[ID], s.smallr);
   const T pr = O::mx(qr.v[IP], rr * s.smallp);
   const T ur = qr.v[ln];
   T ecinr = h * rr * ur * ur;
-  ecinr += h * rr * qr.v[tn] * qr.v[tn];
-  const T etotr = pr * s.entho + ecinr;
+  ecinr = O::mad(h * rr * qr.v[tn], qr.v[tn], ecinr);
+  const T etotr = O::mad(pr, s.entho, ecinr);
   const T ptotr = pr;
 
   const T cfastl = floored_sqrt<FS>(O::mx(O::div(s.gamma0 * pl, rl), s.smallc2));
@@ -255,8 +261,11 @@ __device__ __forceinline__ void hllc(const SchemeT<T> &s, const Q4T<T> &ql, cons
 
   const T rcl = rl * (ul - SL);
   const T rcr = rr * (SR - ur);
+  // separate products: for mirrored states at a reflecting wall they cancel
+  // exactly, so ustar and the wall's mass flux are exactly zero
   const T ustar = O::div(rcr * ur + rcl * ul + (ptotl - ptotr), rcr + rcl);
-  const T ptotstar = O::div(rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur), rcr + rcl);
+  const T ptotstar =
+      O::div(O::mad(rcl * rcr, ul - ur, O::mad(rcl, ptotr, rcr * ptotl)), rcr + rcl);
 
   const bool left = ustar > T(0.0);
   const T rk = left ? rl : rr;
@@ -265,7 +274,7 @@ __device__ __forceinline__ void hllc(const SchemeT<T> &s, const Q4T<T> &ql, cons
   const T ek = left ? etotl : etotr;
   const T pk = left ? ptotl : ptotr;
   const T rstar = O::div(rk * (Sk - uk), Sk - ustar);
-  const T etotstar = O::div((Sk - uk) * ek - pk * uk + ptotstar * ustar, Sk - ustar);
+  const T etotstar = O::div(O::mad(ptotstar, ustar, O::mad(-pk, uk, (Sk - uk) * ek)), Sk - ustar);
 
   T ro, uo, ptoto, etoto;
   if (SL > T(0.0)) {
@@ -277,7 +286,7 @@ __device__ __forceinline__ void hllc(const SchemeT<T> &s, const Q4T<T> &ql, cons
   }
   const T fd = ro * uo;
   flux[ID] = fd;
-  flux[ln] = ro * uo * uo + ptoto;
+  flux[ln] = O::mad(ro * uo, uo, ptoto);
   flux[IP] = (etoto + ptoto) * uo;
   flux[tn] = fd * ((fd > T(0.0)) ? ql.v[tn] : qr.v[tn]);
 }
@@ -296,8 +305,8 @@ __device__ __forceinline__ void hll(const SchemeT<T> &s, const Q4T<T> &ql, const
   const T rr = O::mx(qr.v[ID], s.smallr);
   const T pr = O::mx(qr.v[IP], rr * s.smallp);
   const T ur = qr.v[ln], vr = qr.v[tn];
-  const T el = pl * s.entho + (h * rl * ul * ul + h * rl * vl * vl);
-  const T er = pr * s.entho + (h * rr * ur * ur + h * rr * vr * vr);
+  const T el = O::mad(pl, s.entho, O::mad(h * rl * vl, vl, h * rl * ul * ul));
+  const T er = O::mad(pr, s.entho, O::mad(h * rr * vr, vr, h * rr * ur * ur));
   const T cfastl = floored_sqrt<FS>(O::mx(O::div(s.gamma0 * pl, rl), s.smallc2));
   const T cfastr = floored_sqrt<FS>(O::mx(O::div(s.gamma0 * pr, rr), s.smallc2));
   const T SL = O::mn(ul, ur) - O::mx(cfastl, cfastr);
@@ -305,8 +314,8 @@ __device__ __forceinline__ void hll(const SchemeT<T> &s, const Q4T<T> &ql, const
   // variable order of the conserved state: rho, E, normal momentum, transverse momentum
   const T UL[4] = {rl, el, rl * ul, rl * vl};
   const T UR[4] = {rr, er, rr * ur, rr * vr};
-  const T FL[4] = {rl * ul, (el + pl) * ul, rl * ul * ul + pl, rl * ul * vl};
-  const T FR[4] = {rr * ur, (er + pr) * ur, rr * ur * ur + pr, rr * ur * vr};
+  const T FL[4] = {rl * ul, (el + pl) * ul, O::mad(rl * ul, ul, pl), rl * ul * vl};
+  const T FR[4] = {rr * ur, (er + pr) * ur, O::mad(rr * ur, ur, pr), rr * ur * vr};
   T f[4];
   if (SL >= T(0.0)) {
     for (int k = 0; k < 4; k++) f[k] = FL[k];
@@ -314,7 +323,10 @@ __device__ __forceinline__ void hll(const SchemeT<T> &s, const Q4T<T> &ql, const
     for (int k = 0; k < 4; k++) f[k] = FR[k];
   } else {
     const T inv = O::div(T(1.0), SR - SL);
-    for (int k = 0; k < 4; k++) f[k] = (SR * FL[k] - SL * FR[k] + SL * SR * (UR[k] - UL[k])) * inv;
+    // SR*FL - SL*FR with separate products: exactly zero mass flux for
+    // mirrored states at a reflecting wall
+    for (int k = 0; k < 4; k++)
+      f[k] = O::mad(SL * SR, UR[k] - UL[k], SR * FL[k] - SL * FR[k]) * inv;
   }
   flux[ID] = f[0];
   flux[IP] = f[1];
