@@ -148,7 +148,7 @@ def split_comparison(params, run64, steps, cells, fp64_ms, tile=0) -> dict:
     if tile > 0:
         run.set_tile(tile)
     else:
-        run.autotune(steps=10)
+        run.autotune(steps=20)
     run.run(20)
     run.synchronize()
     run.init_problem()
@@ -202,7 +202,7 @@ def main() -> int:
     if args.tile > 0:
         run.set_tile(args.tile)
     elif args.calibrate_copy <= 0:
-        run.autotune(steps=10)
+        run.autotune(steps=20)
 
     if args.calibrate_copy > 0:
         from euler2d_kokkos_amd import E2D_U, E2D_U2

@@ -264,8 +264,9 @@ __device__ __forceinline__ void hllc(const SchemeT<T> &s, const Q4T<T> &ql, cons
   // separate products: for mirrored states at a reflecting wall they cancel
   // exactly, so ustar and the wall's mass flux are exactly zero
   const T ustar = O::div(rcr * ur + rcl * ul + (ptotl - ptotr), rcr + rcl);
-  const T ptotstar =
-      O::div(O::mad(rcl * rcr, ul - ur, O::mad(rcl, ptotr, rcr * ptotl)), rcr + rcl);
+  // the left/right-symmetric sum stays unfused, so that mirrored states give
+  // mirrored fluxes in FP32 too
+  const T ptotstar = O::div(O::mad(rcl * rcr, ul - ur, rcr * ptotl + rcl * ptotr), rcr + rcl);
 
   const bool left = ustar > T(0.0);
   const T rk = left ? rl : rr;

@@ -198,7 +198,8 @@ int e2d_get_schedule(e2d_run *run, int *schedule);
 int e2d_set_tile(e2d_run *run, int seg_rows);
 int e2d_get_tile(e2d_run *run, int *seg_rows);
 /* Times `steps` loop iterations for each candidate seg_rows on the current
- * state (restored afterwards, t and nStep untouched) and keeps the fastest.
+ * state (restored afterwards, t and nStep untouched), in two interleaved
+ * passes, and keeps the candidate with the fastest timing.
  * candidates == NULL or ncand <= 0: a fixed set plus, for k = 1..4, the
  * smallest tile whose wavefronts fit k per SIMD of the device. */
 int e2d_autotune(e2d_run *run, const int *candidates, int ncand, int steps, int *best_seg_rows,
@@ -257,9 +258,9 @@ int e2d_batch_status(e2d_batch *batch, int instance, double *t, int *nStep);
 int e2d_batch_copy_state(e2d_batch *batch, int instance, double *host, int layout);
 /* total interior cells, wavefronts per step, instance count */
 int e2d_batch_info(e2d_batch *batch, long long *cells, int *waves, int *count);
-/* Times each candidate seg_rows over `steps` batched steps (restoring the
- * state afterwards is the caller's business: call e2d_batch_init) and keeps the
- * fastest; returns it. */
+/* Times each candidate seg_rows over `steps` batched steps, in two
+ * interleaved passes (restoring the state afterwards is the caller's business:
+ * call e2d_batch_init), and keeps the fastest; returns it. */
 int e2d_batch_autotune(e2d_batch *batch, const int *candidates, int ncand, int steps,
                        int *best_seg_rows, double *best_ms_per_step);
 
