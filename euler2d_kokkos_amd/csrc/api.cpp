@@ -559,6 +559,8 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) return fail(E2D_ERR_HIP, "no HIP device visible");
     auto *r = new e2d_run();
+    // every failure past this point releases what the handle holds
+    const int rc1 = [&]() -> int {
     r->p = *p;
     r->kp = e2d::make_kparams(*p, precision);
     r->precision = precision;
@@ -569,11 +571,7 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
     else if (precision == E2D_PRECISION_FP64 && p->implementationVersion == 2)
       r->schedule = E2D_SCHEDULE_ATOMIC;
     HIP_TRY(hipGetDevice(&r->device));
-    e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-      delete r;
-      return hip_fail(e, "hipStreamCreate");
-    }
+    HIP_TRY(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     void *ptr = nullptr;
     int rc2 = alloc_grid(r, r->U, precision);
     if (!rc2) rc2 = alloc_grid(r, r->U2, precision);
@@ -587,11 +585,7 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
     if (!rc2) r->partials = static_cast<double *>(ptr);
     if (!rc2) rc2 = alloc_dev(r, &ptr, 4 * sizeof(double));
     if (!rc2) r->sums4 = static_cast<double *>(ptr);
-    if (rc2) {
-      const std::string msg = g_last_error;
-      e2d_destroy(r);
-      return fail(rc2, msg);
-    }
+    if (rc2) return rc2;
     // 32-bit row offsets in the step kernels need planes of at most 4 GiB;
     // E2D_WIDE_ADDRESSING=1 selects the 64-bit form for any size
     const char *wide = std::getenv("E2D_WIDE_ADDRESSING");
@@ -605,6 +599,13 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
     HIP_TRY(hipMemsetAsync(r->U2.p, 0, bytes, r->stream));
     HIP_TRY(hipMemsetAsync(r->ctl, 0, sizeof(e2d::StepCtl), r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
+    return E2D_OK;
+    }();
+    if (rc1) {
+      const std::string msg = g_last_error;
+      e2d_destroy(r);
+      return fail(rc1, msg);
+    }
     *out = r;
     return E2D_OK;
   });
@@ -864,15 +865,19 @@ int e2d_autotune(e2d_run *r, const int *candidates, int ncand, int steps, int *b
   }
   for (int c = 0; c < ncand; ++c)
     if (candidates[c] < 1 || candidates[c] > 4096) return fail(E2D_ERR_ARG, "bad candidate");
-  const int schedule = r->schedule;
-  r->schedule = E2D_SCHEDULE_FUSED;
-  // snapshot of both arrays; the tuning runs advance a throw-away copy of the loop
+  // snapshot of both arrays; the tuning runs advance a throw-away copy of the
+  // loop.  Every exit below releases the snapshots and restores the schedule.
   Grid s1{}, s2{};
   rc = alloc_grid(r, s1, r->precision);
   if (!rc) rc = alloc_grid(r, s2, r->precision);
-  if (rc) return rc;
-  HIP_TRY(e2d::launch_copy(r->kp, r->U, s1, r->stream));
-  HIP_TRY(e2d::launch_copy(r->kp, r->U2, s2, r->stream));
+  hipError_t e = hipSuccess;
+  if (!rc) {
+    e = e2d::launch_copy(r->kp, r->U, s1, r->stream);
+    if (e == hipSuccess) e = e2d::launch_copy(r->kp, r->U2, s2, r->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "launch_copy(snapshot)");
+  }
+  const int schedule = r->schedule;
+  r->schedule = E2D_SCHEDULE_FUSED;
   const double timers1 = r->timers[1];
   const int tile0 = r->seg_rows;
   int best = tile0;
@@ -893,8 +898,9 @@ int e2d_autotune(e2d_run *r, const int *candidates, int ncand, int steps, int *b
         rc = run_loop_impl(r, steps, &t, &n, &ms, &ex);
       }
       if (!rc && ex > 0) cand_ms[(size_t)c] = std::min(cand_ms[(size_t)c], ms / ex);
-      HIP_TRY(e2d::launch_copy(r->kp, s1, r->U, r->stream));
-      HIP_TRY(e2d::launch_copy(r->kp, s2, r->U2, r->stream));
+      e = e2d::launch_copy(r->kp, s1, r->U, r->stream);
+      if (e == hipSuccess) e = e2d::launch_copy(r->kp, s2, r->U2, r->stream);
+      if (e != hipSuccess && !rc) rc = hip_fail(e, "launch_copy(restore)");
     }
   }
   for (int c = 0; c < ncand; ++c)
@@ -902,9 +908,11 @@ int e2d_autotune(e2d_run *r, const int *candidates, int ncand, int steps, int *b
       best_ms = cand_ms[(size_t)c];
       best = candidates[c];
     }
-  HIP_TRY(hipStreamSynchronize(r->stream));
+  e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
   // release the snapshot arrays
   for (void *ptr : {s1.p, s2.p}) {
+    if (!ptr) continue;
     for (auto it = r->allocs.begin(); it != r->allocs.end(); ++it)
       if (*it == ptr) {
         r->allocs.erase(it);
@@ -1078,12 +1086,17 @@ int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_bat
     b->insts = static_cast<e2d::BatchInst *>(p1);
     b->ctls = static_cast<e2d::StepCtl *>(p2);
     b->blocks_done = static_cast<unsigned int *>(p3);
-    HIP_TRY(hipMemsetAsync(b->ctls, 0, sizeof(e2d::StepCtl) * count, b->stream));
-    HIP_TRY(hipMemsetAsync(b->blocks_done, 0, 64, b->stream));
+    e = hipMemsetAsync(b->ctls, 0, sizeof(e2d::StepCtl) * count, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->blocks_done, 0, 64, b->stream);
+    if (e != hipSuccess) {
+      e2d_batch_destroy(b);
+      return hip_fail(e, "hipMemsetAsync(batch tables)");
+    }
     int rc = batch_upload_table(b);
     if (rc) {
+      const std::string msg = g_last_error;
       e2d_batch_destroy(b);
-      return rc;
+      return fail(rc, msg);
     }
     *out = b;
     return E2D_OK;
