@@ -10,6 +10,7 @@
 // instantiation computes in FP32 on a hi+lo FP32 conserved state.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 
@@ -761,8 +762,11 @@ struct MarchState {
 
 // One iteration over loaded row r.  STAGES is a compile-time set, so the
 // steady-state iteration is one branch-free basic block in which the compiler
-// can interleave the independent x-face and y-face Riemann chains.
-template <int RS, class P, int STAGES, bool PREFETCH, bool RATE, bool O32, bool EDGE>
+// can interleave the independent x-face and y-face Riemann chains.  COLS: the
+// wavefront covers ghost columns (flips and ghost-column stores); ROWS: rows
+// r - 2 .. r + 1 may be ghost rows or lie past the last row (row ghost map,
+// flips and stores).  Without ROWS, rows r + 1 <= ny + 1 and 2 <= r.
+template <int RS, class P, int STAGES, bool PREFETCH, bool RATE, bool O32, bool COLS, bool ROWS>
 __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, const Grid &out,
                                           MarchState<P> &m, int r, int c0, bool flipx,
                                           typename P::T dtdx, typename P::T dtdy,
@@ -772,15 +776,18 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   using Cell = typename P::Cell;
   const SchemeT<T> &s = P::scheme(kp);
   const Cell uc = m.un;
-  if constexpr (PREFETCH && EDGE) {
-    bool flipy;
-    const int rc = min(r + 1, kp.ny + 3);
-    const int r0 = ghost_map_uniform(rc, kp.ny, kp.bcymin, kp.bcymax, flipy);
-    m.un = P::template load_rc<O32>(in, r0, pin_offset((unsigned)c0 * sizeof(T)));
-    if (flipx) P::negate(m.un, IU);
-    if (flipy) P::negate(m.un, IV);
-  } else if constexpr (PREFETCH) {  // interior wavefront: row r + 1 is inside the domain
-    m.un = P::template load_rc<O32>(in, r + 1, pin_offset((unsigned)c0 * sizeof(T)));
+  if constexpr (PREFETCH) {
+    bool flipy = false;
+    int rn = r + 1;
+    if constexpr (ROWS)
+      rn = ghost_map_uniform(min(r + 1, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
+    m.un = P::template load_rc<O32>(in, rn, pin_offset((unsigned)c0 * sizeof(T)));
+    if constexpr (COLS) {
+      if (flipx) P::negate(m.un, IU);
+    }
+    if constexpr (ROWS) {
+      if (flipy) P::negate(m.un, IV);
+    }
   }
   Q4T<T> q0;
   P::prim(kp, uc, q0);
@@ -823,9 +830,15 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
 
   // stores last, so that the arithmetic above stays in one basic block
   // ghost cells of row r owned by this wave: materialise them in both arrays
-  if constexpr (EDGE) {
+  if constexpr (ROWS) {
     const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
     if ((ghost_col || row_ghost) && r >= row_lo && r <= row_hi) {
+      const unsigned cb = pin_offset((unsigned)col * sizeof(T));
+      P::template store_rc<O32>(in, r, cb, uc);
+      P::template store_rc<O32>(out, r, cb, uc);
+    }
+  } else if constexpr (COLS) {  // row r is an interior row: ghost columns only
+    if (ghost_col && r >= row_lo && r <= row_hi) {
       const unsigned cb = pin_offset((unsigned)col * sizeof(T));
       P::template store_rc<O32>(in, r, cb, uc);
       P::template store_rc<O32>(out, r, cb, uc);
@@ -833,7 +846,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   }
   if constexpr ((STAGES & kUpdate) != 0) {
     const int b = r - 2;
-    if (out_col && (!EDGE || b <= kp.ny + 1)) {
+    if (out_col && (!ROWS || b <= kp.ny + 1)) {
       P::template store_rc<O32>(out, b, pin_offset((unsigned)col * sizeof(T)), unew);
       if constexpr (RATE) {  // next-step CFL rate of the new state
         Q4T<T> qn;
@@ -860,7 +873,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
 // slopes of row j_base-1; k = 3 the first y and x faces; k = 4 .. seg_rows+2
 // is the steady state (all stages, branch-free); k = seg_rows+3 finishes the
 // last y face and update.
-template <int RS, class P, bool RATE, bool O32, bool EDGE>
+template <int RS, class P, bool RATE, bool O32, bool COLS>
 __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in, const Grid &out,
                                                double dt, int seg_rows, int nstrips, int strip,
                                                int seg) {
@@ -874,40 +887,39 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
   const int col = i_base - 2 + lane;
   const int j_base = 2 + seg_rows * seg;
 
-  // an interior wavefront (EDGE = false) loads only cells inside the domain,
-  // owns no ghost cell and updates valid rows only (interior_wave())
+  // A wavefront with COLS = false loads only interior columns and owns no
+  // ghost column (interior_cols()).
   bool flipx = false;
   int c0 = col;
-  bool out_col = lane >= 2 && lane < 2 + kOutCols;
+  bool col_owned = lane >= 2 && lane < 2 + kOutCols;
   bool ghost_col = false;
-  int wlo = 1, whi = -1;
-  if constexpr (EDGE) {
+  if constexpr (COLS) {
     const int colc = min(col, kp.nx + 3);
     c0 = ghost_map(colc, kp.nx, kp.bcxmin, kp.bcxmax, flipx);
     const bool col_ghost = col < 2 || (col > kp.nx + 1 && col <= kp.nx + 3);
-    const bool col_owned = (lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1) ||
-                           (strip == 0 && lane < 2) ||
-                           (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
-    out_col = out_col && col <= kp.nx + 1;
-    // Ghost cells this wave writes: owned column, owned row, and a ghost column or
-    // ghost row.  The owned rows of a segment are contiguous: [row_lo, row_hi].
-    const int row_lo = seg == 0 ? 0 : j_base;
-    const int row_hi = seg == nsegs - 1 ? kp.ny + 3 : min(j_base + seg_rows - 1, kp.ny + 1);
+    col_owned = (lane >= 2 && lane < 2 + kOutCols && col <= kp.nx + 1) ||
+                (strip == 0 && lane < 2) ||
+                (strip == nstrips - 1 && col > kp.nx + 1 && col <= kp.nx + 3);
     ghost_col = col_ghost && col_owned;
-    wlo = col_owned ? row_lo : 1;  // empty range for columns not owned
-    whi = col_owned ? row_hi : -1;
   }
+  const bool out_col = lane >= 2 && lane < 2 + kOutCols && (!COLS || col <= kp.nx + 1);
+  // Ghost cells this wave writes: owned column, owned row, and a ghost column or
+  // ghost row.  The owned rows of a segment are contiguous: [row_lo, row_hi].
+  const int row_lo = seg == 0 ? 0 : j_base;
+  const int row_hi = seg == nsegs - 1 ? kp.ny + 3 : min(j_base + seg_rows - 1, kp.ny + 1);
+  const int wlo = col_owned ? row_lo : 1;  // empty range for columns not owned
+  const int whi = col_owned ? row_hi : -1;
 
   MarchState<P> m;
-  if constexpr (EDGE) {
+  {
     bool flipy;
-    const int r0 =
+    const int rr =
         ghost_map_uniform(min(j_base - 2, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
-    m.un = P::template load_rc<O32>(in, r0, (unsigned)c0 * sizeof(T));
-    if (flipx) P::negate(m.un, IU);
+    m.un = P::template load_rc<O32>(in, rr, (unsigned)c0 * sizeof(T));
+    if constexpr (COLS) {
+      if (flipx) P::negate(m.un, IU);
+    }
     if (flipy) P::negate(m.un, IV);
-  } else {
-    m.un = P::template load_rc<O32>(in, j_base - 2, (unsigned)c0 * sizeof(T));
   }
   m.ua = m.un;
   m.ub = m.un;
@@ -915,23 +927,30 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
   m.wmax = T(0.0);
 
   const int r0 = j_base - 2;
-#define E2D_ROW(ST, PF, RT, k)                                                               \
-  march_row<RS, P, ST, PF, RT, O32, EDGE>(kp, in, out, m, r0 + (k), c0, flipx, dtdx, dtdy,    \
-                                          ghost_col, out_col, col, wlo, whi)
-  E2D_ROW(0, true, false, 0);
-  E2D_ROW(0, true, false, 1);
-  E2D_ROW(kSlopes, true, false, 2);
-  E2D_ROW(kSlopes | kYFace | kXFace, true, false, 3);
-  // steady state, unrolled three times: the rotating row buffers of MarchState
-  // are then renamed instead of copied at every iteration
+  // rows this segment updates: seg_rows, fewer in a top segment that ends at
+  // row ny + 1 (its ghost rows ny + 2, ny + 3 are the last two loaded)
+  const int rows = min(seg_rows, kp.ny + 2 - j_base);
+#define E2D_ROW(ST, PF, RT, RW, k)                                                           \
+  march_row<RS, P, ST, PF, RT, O32, COLS, RW>(kp, in, out, m, r0 + (k), c0, flipx, dtdx,      \
+                                              dtdy, ghost_col, out_col, col, wlo, whi)
+  E2D_ROW(0, true, false, true, 0);
+  E2D_ROW(0, true, false, true, 1);
+  E2D_ROW(kSlopes, true, false, true, 2);
+  E2D_ROW(kSlopes | kYFace | kXFace, true, false, true, 3);
+  // Steady state.  Iterations k <= ny - r0 load row r + 1 <= ny + 1, and from
+  // k = 4 on row r >= 4: no row ghost handling.  They are unrolled three
+  // times, so the rotating row buffers of MarchState are renamed instead of
+  // copied.  Only the top segment has iterations past that bound.
+  const int k_rows = min(rows + 2, kp.ny - r0);
   int k = 4;
-  for (; k + 2 <= seg_rows + 2; k += 3) {
-    E2D_ROW(kAll, true, RATE, k);
-    E2D_ROW(kAll, true, RATE, k + 1);
-    E2D_ROW(kAll, true, RATE, k + 2);
+  for (; k + 2 <= k_rows; k += 3) {
+    E2D_ROW(kAll, true, RATE, false, k);
+    E2D_ROW(kAll, true, RATE, false, k + 1);
+    E2D_ROW(kAll, true, RATE, false, k + 2);
   }
-  for (; k <= seg_rows + 2; ++k) E2D_ROW(kAll, true, RATE, k);
-  E2D_ROW(kSlopes | kYFace | kUpdate, false, RATE, seg_rows + 3);
+  for (; k <= k_rows; ++k) E2D_ROW(kAll, true, RATE, false, k);
+  for (; k <= rows + 2; ++k) E2D_ROW(kAll, true, RATE, true, k);
+  E2D_ROW(kSlopes | kYFace | kUpdate, false, RATE, true, rows + 3);
 #undef E2D_ROW
   return m.wmax;
 }
@@ -994,15 +1013,29 @@ __device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long lon
   return last;
 }
 
-// True when every row and column the wavefront of (strip, seg) loads lies
-// inside the domain: it then maps, flips and writes no ghost cell, and every
-// row it updates is an interior row.  Columns loaded: 60 strip .. 60 strip + 63;
-// rows loaded: seg_rows seg .. seg_rows (seg + 1) + 3.
-__device__ __forceinline__ bool interior_wave(const KParams &kp, int seg_rows, int strip,
-                                              int seg) {
-  const int c_lo = kOutCols * strip, r_lo = seg_rows * seg;
-  return c_lo >= 2 && c_lo + kLanes - 1 <= kp.nx + 1 && r_lo >= 2 &&
-         r_lo + seg_rows + 3 <= kp.ny + 1;
+// True when every column the wavefront of strip `strip` loads (60 strip ..
+// 60 strip + 63) lies inside the domain: it then maps, flips and writes no
+// ghost column.
+__device__ __forceinline__ bool interior_cols(const KParams &kp, int strip) {
+  const int c_lo = kOutCols * strip;
+  return c_lo >= 2 && c_lo + kLanes - 1 <= kp.nx + 1;
+}
+
+// Wavefront -> (strip, segment, tile) of the fused step: the wavefronts of the
+// interior strips first, then those of the edge strips (0 and nstrips - 1).
+__device__ __forceinline__ void fused_wave(const FusedMap &f, int wave, int &strip, int &seg,
+                                           int &tile) {
+  if (wave < f.n_int) {
+    const int ni = f.nstrips - 2;  // n_int > 0 only with more than two strips
+    strip = 1 + wave % ni;
+    seg = wave / ni;
+    tile = f.tile;
+  } else {
+    const int w = wave - f.n_int;
+    strip = w % f.nedge == 0 ? 0 : f.nstrips - 1;
+    seg = w / f.nedge;
+    tile = f.tile_e;
+  }
 }
 
 // LOOP = false: explicit dt (HydroRun::godunov_unsplit); true: run-loop mode.
@@ -1010,7 +1043,7 @@ __device__ __forceinline__ bool interior_wave(const KParams &kp, int seg_rows, i
 template <int RS, class P, bool LOOP, bool O32>
 __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in, Grid out,
                                                             double dt_arg, StepCtl *ctl,
-                                                            int seg_rows, int nstrips, int nwaves) {
+                                                            FusedMap fm) {
   using T = typename P::T;
   if (LOOP && !ctl->active) return;  // loop finished: nothing to do (uniform)
   // the wave index is uniform: every row index and the row ghost map derived
@@ -1018,13 +1051,14 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
   const int wave =
       __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6));
   T wmax = T(0.0);
-  if (wave < nwaves) {
-    const int strip = wave % nstrips, seg = wave / nstrips;
+  if (wave < fm.nwaves) {
+    int strip, seg, tile;
+    fused_wave(fm, wave, strip, seg, tile);
     const double dt = LOOP ? ctl->dt : dt_arg;
-    if (interior_wave(kp, seg_rows, strip, seg))  // uniform
-      wmax = march<RS, P, LOOP, O32, false>(kp, in, out, dt, seg_rows, nstrips, strip, seg);
+    if (interior_cols(kp, strip))  // uniform
+      wmax = march<RS, P, LOOP, O32, false>(kp, in, out, dt, tile, fm.nstrips, strip, seg);
     else
-      wmax = march<RS, P, LOOP, O32, true>(kp, in, out, dt, seg_rows, nstrips, strip, seg);
+      wmax = march<RS, P, LOOP, O32, true>(kp, in, out, dt, tile, fm.nstrips, strip, seg);
   }
   if (!LOOP) return;
   // run-loop epilogue: next-step invDt (exact max, order independent) and the
@@ -1074,7 +1108,7 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__r
       const Grid gin = even ? bi.U : bi.U2;
       const Grid gout = even ? bi.U2 : bi.U;
       const int strip = w % bi.nstrips, seg = w / bi.nstrips;
-      if (interior_wave(kp, seg_rows, strip, seg))
+      if (interior_cols(kp, strip))
         wmax = march<RS, P, true, O32, false>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips,
                                               strip, seg);
       else
@@ -1201,20 +1235,18 @@ dim3 grid2d(int nx, int ny, dim3 blk) {
 template <int RS, class P>
 void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl, int seg_rows,
                     hipStream_t st) {
-  const int nstrips = (kp.nx + kOutCols - 1) / kOutCols;
-  const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
-  const int nwaves = nstrips * nsegs;
+  const FusedMap fm = fused_map(kp, seg_rows);
   const int waves_per_block = kBlock / kLanes;
-  const int blocks = (nwaves + waves_per_block - 1) / waves_per_block;
+  const int blocks = (fm.nwaves + waves_per_block - 1) / waves_per_block;
   if (ctl && off32_ok(kp, in) && off32_ok(kp, out))
     hipLaunchKernelGGL((step_fused_kernel<RS, P, true, true>), dim3(blocks), dim3(kBlock), 0, st,
-                       kp, in, out, dt, ctl, seg_rows, nstrips, nwaves);
+                       kp, in, out, dt, ctl, fm);
   else if (ctl)
     hipLaunchKernelGGL((step_fused_kernel<RS, P, true, false>), dim3(blocks), dim3(kBlock), 0, st,
-                       kp, in, out, dt, ctl, seg_rows, nstrips, nwaves);
+                       kp, in, out, dt, ctl, fm);
   else
     hipLaunchKernelGGL((step_fused_kernel<RS, P, false, false>), dim3(blocks), dim3(kBlock), 0,
-                       st, kp, in, out, dt, ctl, seg_rows, nstrips, nwaves);
+                       st, kp, in, out, dt, ctl, fm);
 }
 
 template <class P>
@@ -1421,6 +1453,19 @@ hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, St
   else
     launch_fused_p<P64>(kp, in, out, dt, ctl, seg_rows, st);
   return hipGetLastError();
+}
+
+FusedMap fused_map(const KParams &kp, int seg_rows) {
+  FusedMap f{};
+  f.tile = seg_rows;
+  f.tile_e = std::max(1, (seg_rows + 4) * 10 / 11 - 4);
+  f.nstrips = (kp.nx + kOutCols - 1) / kOutCols;
+  f.nedge = std::min(f.nstrips, 2);
+  const int nsegs_i = (kp.ny + f.tile - 1) / f.tile;
+  const int nsegs_e = (kp.ny + f.tile_e - 1) / f.tile_e;
+  f.n_int = (f.nstrips - f.nedge) * nsegs_i;
+  f.nwaves = f.n_int + f.nedge * nsegs_e;
+  return f;
 }
 
 int batch_waves(const KParams &kp, int seg_rows, int *nstrips) {

@@ -75,6 +75,20 @@ constexpr int kLanes = 64;
 constexpr int kOutCols = 60;
 constexpr int kBlock = 256;
 
+// Work map of the fused step.  A wavefront covers one segment of one strip.
+// The edge strips (0 and nstrips - 1) also write ghost columns and take about
+// 10% longer per row, so their segments are shorter: tile_e = (tile + 4) *
+// 10 / 11 - 4 rows, which with the 4 warm-up rows of every segment makes an
+// edge wavefront end with the interior ones.  Interior-strip wavefronts come
+// first in the launch.
+struct FusedMap {
+  int tile, tile_e;  // rows per segment: interior strips, edge strips
+  int nstrips, nedge;  // strips of 60 columns; edge strips (min(nstrips, 2))
+  int n_int;           // wavefronts of the interior strips
+  int nwaves;          // all wavefronts
+};
+FusedMap fused_map(const KParams &kp, int seg_rows);
+
 KParams make_kparams(const e2d_params &p, int precision);
 // planes of one state array and bytes of its storage scalar
 int grid_planes(int precision);
