@@ -842,21 +842,18 @@ int e2d_autotune(e2d_run *r, const int *candidates, int ncand, int steps, int *b
   if (steps <= 0) return fail(E2D_ERR_ARG, "bad argument");
   std::vector<int> auto_cand;
   if (!candidates || ncand <= 0) {
-    // A wavefront alone keeps its SIMD's VALU as busy as two sharing it, so
-    // the step time follows the rows marched per SIMD: the tiles at which
-    // the wavefronts just fit k per SIMD (k = 1..4) are the natural
-    // candidates, next to a fixed set.
+    // A wavefront alone keeps its SIMD's VALU about as busy as two sharing
+    // it, so the step time follows the rows marched per SIMD: the smallest
+    // tiles at which the step's wavefronts fit k per SIMD (k = 1..4) are the
+    // natural candidates, next to a fixed set.
     auto_cand = {8, 12, 14, 16, 20, 24, 32, 48, 64};
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, r->device) == hipSuccess) {
       const int simds = 4 * prop.multiProcessorCount;
-      const int nstrips = (r->p.nx + e2d::kOutCols - 1) / e2d::kOutCols;
       for (int k = 1; k <= 4; ++k) {
-        const int segs = k * simds / nstrips;
-        if (segs < 1) continue;
-        const int tile = (r->p.ny + segs - 1) / segs;
-        if (tile >= 1 && tile <= 4096 &&
-            std::find(auto_cand.begin(), auto_cand.end(), tile) == auto_cand.end())
+        int tile = 1;
+        while (tile < 4096 && e2d::fused_map(r->kp, tile).nwaves > k * simds) ++tile;
+        if (std::find(auto_cand.begin(), auto_cand.end(), tile) == auto_cand.end())
           auto_cand.push_back(tile);
       }
     }
