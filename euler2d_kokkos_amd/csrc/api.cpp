@@ -328,20 +328,18 @@ int download_planes(e2d_run *r, const Grid &g, std::vector<double> &out) {
     HIP_TRY(hipStreamSynchronize(r->stream));
     return E2D_OK;
   }
-  const int nk = e2d::grid_planes(r->precision);
-  std::vector<float> tmp(nk * cells);
-  const float *src = static_cast<const float *>(g.p);
-  for (int k = 0; k < nk; k++)
-    HIP_TRY(hipMemcpy2DAsync(tmp.data() + k * cells, (size_t)isz * sizeof(float),
-                             src + k * g.plane, (size_t)g.pitch * sizeof(float),
-                             (size_t)isz * sizeof(float), (size_t)jsz, hipMemcpyDeviceToHost,
-                             r->stream));
-  HIP_TRY(hipStreamSynchronize(r->stream));
+  // split: (hi, lo) float pairs; FP32: floats
+  const int per = r->precision == E2D_PRECISION_FP32_SPLIT ? 2 : 1;
+  const size_t eb = per * sizeof(float);
+  std::vector<float> tmp((size_t)per * 4 * cells);
+  const char *src = static_cast<const char *>(g.p);
   for (int v = 0; v < 4; v++)
-    for (size_t c = 0; c < cells; c++)
-      out[v * cells + c] = nk == 8 ? (double)tmp[(2 * v) * cells + c] +
-                                         (double)tmp[(2 * v + 1) * cells + c]
-                                   : (double)tmp[v * cells + c];
+    HIP_TRY(hipMemcpy2DAsync(tmp.data() + (size_t)per * v * cells, (size_t)isz * eb,
+                             src + (size_t)v * g.plane * eb, (size_t)g.pitch * eb, (size_t)isz * eb,
+                             (size_t)jsz, hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  for (size_t k = 0; k < 4 * cells; k++)
+    out[k] = per == 2 ? (double)tmp[2 * k] + (double)tmp[2 * k + 1] : (double)tmp[k];
   return E2D_OK;
 }
 
@@ -359,25 +357,24 @@ int upload_planes(e2d_run *r, const Grid &g, const std::vector<double> &in) {
     HIP_TRY(hipStreamSynchronize(r->stream));
     return E2D_OK;
   }
-  const int nk = e2d::grid_planes(r->precision);
-  std::vector<float> tmp(nk * cells);
-  for (int v = 0; v < 4; v++)
-    for (size_t c = 0; c < cells; c++) {
-      const double x = in[v * cells + c];
-      const float hi = (float)x;
-      if (nk == 8) {
-        tmp[(2 * v) * cells + c] = hi;
-        tmp[(2 * v + 1) * cells + c] = (float)(x - (double)hi);
-      } else {
-        tmp[v * cells + c] = hi;
-      }
+  const int per = r->precision == E2D_PRECISION_FP32_SPLIT ? 2 : 1;
+  const size_t eb = per * sizeof(float);
+  std::vector<float> tmp((size_t)per * 4 * cells);
+  for (size_t k = 0; k < 4 * cells; k++) {
+    const double x = in[k];
+    const float hi = (float)x;
+    if (per == 2) {
+      tmp[2 * k] = hi;
+      tmp[2 * k + 1] = (float)(x - (double)hi);
+    } else {
+      tmp[k] = hi;
     }
-  float *dst = static_cast<float *>(g.p);
-  for (int k = 0; k < nk; k++)
-    HIP_TRY(hipMemcpy2DAsync(dst + k * g.plane, (size_t)g.pitch * sizeof(float),
-                             tmp.data() + k * cells, (size_t)isz * sizeof(float),
-                             (size_t)isz * sizeof(float), (size_t)jsz, hipMemcpyHostToDevice,
-                             r->stream));
+  }
+  char *dst = static_cast<char *>(g.p);
+  for (int v = 0; v < 4; v++)
+    HIP_TRY(hipMemcpy2DAsync(dst + (size_t)v * g.plane * eb, (size_t)g.pitch * eb,
+                             tmp.data() + (size_t)per * v * cells, (size_t)isz * eb,
+                             (size_t)isz * eb, (size_t)jsz, hipMemcpyHostToDevice, r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
   return E2D_OK;
 }

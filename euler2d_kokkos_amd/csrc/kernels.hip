@@ -88,6 +88,7 @@ struct RowPtr {
 // FP64: four double planes; the state is the value itself.
 struct P64 {
   using T = double;
+  static constexpr unsigned kElem = sizeof(double);  // bytes per element of a plane
   struct Cell {
     double u[4];
   };
@@ -141,8 +142,9 @@ struct P64 {
   }
 };
 
-// Split precision: hi and lo FP32 planes per variable, about 48 significant
-// bits of state.  Fluxes are FP32.  The update adds the four face fluxes to
+// Split precision: one plane per variable whose elements are (hi, lo) FP32
+// pairs, about 48 significant bits of state; a lane moves both halves with one
+// 8-byte access.  Fluxes are FP32.  The update adds the four face fluxes to
 // hi + lo in FP64, in the reference's order, then stores the FP32 rounding of
 // the sum as hi and the FP32 rounding of the remainder (exact in FP64) as lo.
 // The same FP32 flux value leaves one cell and enters its neighbour, so the
@@ -150,43 +152,43 @@ struct P64 {
 // per update.
 struct PSplit {
   using T = float;
+  static constexpr unsigned kElem = sizeof(float2);
   struct Cell {
     float hi[4], lo[4];
   };
   static __device__ __forceinline__ Cell load(const Grid &g, long long off) {
-    const float *p = static_cast<const float *>(g.p);
+    const float2 *p = static_cast<const float2 *>(g.p);
     Cell c;
     for (int v = 0; v < 4; v++) {
-      c.hi[v] = p[(2 * v) * g.plane + off];
-      c.lo[v] = p[(2 * v + 1) * g.plane + off];
+      const float2 x = p[v * g.plane + off];
+      c.hi[v] = x.x;
+      c.lo[v] = x.y;
     }
     return c;
   }
   static __device__ __forceinline__ void store(const Grid &g, long long off, const Cell &c) {
-    float *p = static_cast<float *>(g.p);
-    for (int v = 0; v < 4; v++) {
-      p[(2 * v) * g.plane + off] = c.hi[v];
-      p[(2 * v + 1) * g.plane + off] = c.lo[v];
-    }
+    float2 *p = static_cast<float2 *>(g.p);
+    for (int v = 0; v < 4; v++) p[v * g.plane + off] = make_float2(c.hi[v], c.lo[v]);
   }
+  // the pair moves as one 64-bit word: low half hi, high half lo
   template <bool O32 = false>
   static __device__ __forceinline__ Cell load_rc(const Grid &g, int row, unsigned cb) {
-    const RowPtr<float, O32> rp(g, row, cb);
+    const RowPtr<unsigned long long, O32> rp(g, row, cb);
     Cell c;
     for (int v = 0; v < 4; v++) {
-      c.hi[v] = *rp.at(g, 2 * v);
-      c.lo[v] = *rp.at(g, 2 * v + 1);
+      const unsigned long long x = *rp.at(g, v);
+      c.hi[v] = __uint_as_float((unsigned)x);
+      c.lo[v] = __uint_as_float((unsigned)(x >> 32));
     }
     return c;
   }
   template <bool O32 = false>
   static __device__ __forceinline__ void store_rc(const Grid &g, int row, unsigned cb,
                                                   const Cell &c) {
-    const RowPtr<float, O32> rp(g, row, cb);
-    for (int v = 0; v < 4; v++) {
-      *rp.at(g, 2 * v) = c.hi[v];
-      *rp.at(g, 2 * v + 1) = c.lo[v];
-    }
+    const RowPtr<unsigned long long, O32> rp(g, row, cb);
+    for (int v = 0; v < 4; v++)
+      *rp.at(g, v) = ((unsigned long long)__float_as_uint(c.lo[v]) << 32) |
+                     __float_as_uint(c.hi[v]);
   }
   static __device__ __forceinline__ void negate(Cell &c, int v) {
     c.hi[v] = -c.hi[v];
@@ -242,6 +244,7 @@ struct PSplit {
 // the split path, not a drop-in.
 struct P32 {
   using T = float;
+  static constexpr unsigned kElem = sizeof(float);
   struct Cell {
     float u[4];
   };
@@ -781,7 +784,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     int rn = r + 1;
     if constexpr (ROWS)
       rn = ghost_map_uniform(min(r + 1, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
-    m.un = P::template load_rc<O32>(in, rn, pin_offset((unsigned)c0 * sizeof(T)));
+    m.un = P::template load_rc<O32>(in, rn, pin_offset((unsigned)c0 * P::kElem));
     if constexpr (COLS) {
       if (flipx) P::negate(m.un, IU);
     }
@@ -833,13 +836,13 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   if constexpr (ROWS) {
     const bool row_ghost = r < 2 || (r > kp.ny + 1 && r <= kp.ny + 3);
     if ((ghost_col || row_ghost) && r >= row_lo && r <= row_hi) {
-      const unsigned cb = pin_offset((unsigned)col * sizeof(T));
+      const unsigned cb = pin_offset((unsigned)col * P::kElem);
       P::template store_rc<O32>(in, r, cb, uc);
       P::template store_rc<O32>(out, r, cb, uc);
     }
   } else if constexpr (COLS) {  // row r is an interior row: ghost columns only
     if (ghost_col && r >= row_lo && r <= row_hi) {
-      const unsigned cb = pin_offset((unsigned)col * sizeof(T));
+      const unsigned cb = pin_offset((unsigned)col * P::kElem);
       P::template store_rc<O32>(in, r, cb, uc);
       P::template store_rc<O32>(out, r, cb, uc);
     }
@@ -847,7 +850,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   if constexpr ((STAGES & kUpdate) != 0) {
     const int b = r - 2;
     if (out_col && (!ROWS || b <= kp.ny + 1)) {
-      P::template store_rc<O32>(out, b, pin_offset((unsigned)col * sizeof(T)), unew);
+      P::template store_rc<O32>(out, b, pin_offset((unsigned)col * P::kElem), unew);
       if constexpr (RATE) {  // next-step CFL rate of the new state
         Q4T<T> qn;
         P::prim(kp, unew, qn);
@@ -915,7 +918,7 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
     bool flipy;
     const int rr =
         ghost_map_uniform(min(j_base - 2, kp.ny + 3), kp.ny, kp.bcymin, kp.bcymax, flipy);
-    m.un = P::template load_rc<O32>(in, rr, (unsigned)c0 * sizeof(T));
+    m.un = P::template load_rc<O32>(in, rr, (unsigned)c0 * P::kElem);
     if constexpr (COLS) {
       if (flipx) P::negate(m.un, IU);
     }
@@ -1224,7 +1227,7 @@ __global__ void copy_kernel(const double2 *__restrict__ src, double2 *__restrict
 // 32-bit row offsets reach every element of a plane of at most 4 GiB
 bool off32_ok(const KParams &kp, const Grid &g) {
   return !kp.wide_addr && (unsigned long long)g.plane *
-                                  (unsigned long long)(kp.prec == E2D_PRECISION_FP64 ? 8 : 4) <=
+                                  (unsigned long long)grid_scalar_bytes(kp.prec) <=
                               (1ull << 32);
 }
 
@@ -1271,8 +1274,8 @@ void launch_fused_p(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
 
 }  // namespace
 
-int grid_planes(int precision) { return precision == E2D_PRECISION_FP32_SPLIT ? 8 : 4; }
-int grid_scalar_bytes(int precision) { return precision == E2D_PRECISION_FP64 ? 8 : 4; }
+int grid_planes(int) { return 4; }
+int grid_scalar_bytes(int precision) { return precision == E2D_PRECISION_FP32 ? 4 : 8; }
 
 KParams make_kparams(const e2d_params &p, int precision) {
   KParams k{};

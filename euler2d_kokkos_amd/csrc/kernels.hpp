@@ -13,8 +13,9 @@ namespace e2d {
 // One state array on the device, as SoA planes [plane][j][i] with a row pitch
 // and a plane stride in elements of the storage scalar:
 //   FP64        4 planes of double, plane v = variable v
-//   FP32 split  8 planes of float, plane 2v = hi part, 2v+1 = lo part of
-//               variable v (value = hi + lo, |lo| <= ulp(hi)/2)
+//   FP32 split  4 planes of (hi, lo) float pairs, plane v = variable v
+//               (value = hi + lo, |lo| <= ulp(hi)/2; hi at the lower address)
+//   FP32        4 planes of float
 struct Grid {
   void *p;
   int pitch;
@@ -90,7 +91,8 @@ struct FusedMap {
 FusedMap fused_map(const KParams &kp, int seg_rows);
 
 KParams make_kparams(const e2d_params &p, int precision);
-// planes of one state array and bytes of its storage scalar
+// planes of one state array (one per conserved variable) and bytes of one
+// element: a double (FP64), a (hi, lo) pair of floats (split) or a float
 int grid_planes(int precision);
 int grid_scalar_bytes(int precision);
 
