@@ -64,6 +64,7 @@ struct e2d_run {
   hipStream_t stream = nullptr;
   Grid U{}, U2{}, Q{}, Fx{}, Fy{}, Sx{}, Sy{};
   e2d::StepCtl *ctl = nullptr;
+  int phase = 0;  // phase of the next run-loop launch in the control rings
   unsigned long long *bits = nullptr;
   unsigned int *count = nullptr;
   double *partials = nullptr;
@@ -174,7 +175,7 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
     if (timed) HIP_TRY(hipEventRecord(ev[4], r->stream));
   } else {
     if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
-    HIP_TRY(e2d::launch_step_fused(r->kp, in, out, dt, nullptr, r->seg_rows, r->stream));
+    HIP_TRY(e2d::launch_step_fused(r->kp, in, out, dt, nullptr, 0, r->seg_rows, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[4], r->stream));
   }
   if (timed) {
@@ -204,14 +205,17 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
 
 // Host-side loop state -> control block; dt of the pending step computed on
 // the device from the current state.
+// Phase 0 of the control rings: the state before the loop's first step and
+// the CFL rate of its input; every other entry cleared.
 int loop_begin(e2d_run *r, double t, int nStep) {
-  e2d::StepCtl c{};
-  c.t = t;
-  c.nstep = nStep;
-  HIP_TRY(hipMemcpyAsync(r->ctl, &c, sizeof(c), hipMemcpyHostToDevice, r->stream));
-  HIP_TRY(e2d::launch_dt_reduce(r->kp, (nStep % 2 == 0) ? r->U : r->U2, &r->ctl->invdt,
+  r->phase = 0;
+  HIP_TRY(hipMemsetAsync(r->ctl, 0, sizeof(e2d::StepCtl), r->stream));
+  e2d::StepState s{};
+  s.t = t;
+  s.nstep = nStep;
+  HIP_TRY(hipMemcpyAsync(&r->ctl->s[0], &s, sizeof(s), hipMemcpyHostToDevice, r->stream));
+  HIP_TRY(e2d::launch_dt_reduce(r->kp, (nStep % 2 == 0) ? r->U : r->U2, &r->ctl->rate[0][0],
                                 r->stream));
-  HIP_TRY(e2d::launch_dt_finalize(r->kp, r->ctl, r->stream));
   return E2D_OK;
 }
 
@@ -220,14 +224,34 @@ int loop_steps(e2d_run *r, int nStep0, int count) {
     const int n = nStep0 + k;
     Grid in = (n % 2 == 0) ? r->U : r->U2;
     Grid out = (n % 2 == 0) ? r->U2 : r->U;
-    HIP_TRY(e2d::launch_step_fused(r->kp, in, out, 0.0, r->ctl, r->seg_rows, r->stream));
+    HIP_TRY(e2d::launch_step_fused(r->kp, in, out, 0.0, r->ctl, r->phase, r->seg_rows,
+                                   r->stream));
+    r->phase = r->phase == 2 ? 0 : r->phase + 1;
   }
   return E2D_OK;
 }
 
-int loop_read(e2d_run *r, e2d::StepCtl &c) {
-  HIP_TRY(hipMemcpyAsync(&c, r->ctl, sizeof(c), hipMemcpyDeviceToHost, r->stream));
+// Loop state after the launched steps, with the while condition of main.cpp:100.
+struct LoopView {
+  double t = 0.0;
+  int nstep = 0;
+  bool active = false;
+};
+
+LoopView loop_view(const e2d_params &p, const e2d::StepState &s) {
+  LoopView v;
+  v.t = s.t;
+  v.nstep = s.nstep;
+  v.active = s.t < p.tEnd && s.nstep < p.nStepmax;
+  return v;
+}
+
+int loop_read(e2d_run *r, LoopView &v) {
+  e2d::StepState s{};
+  HIP_TRY(hipMemcpyAsync(&s, &r->ctl->s[r->phase], sizeof(s), hipMemcpyDeviceToHost,
+                         r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
+  v = loop_view(r->p, s);
   return E2D_OK;
 }
 
@@ -281,7 +305,7 @@ int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elap
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, r->stream));
-  e2d::StepCtl c{};
+  LoopView c;
   int launched = 0;
   for (;;) {
     // with an explicit step budget every launch is queued at once (steps past
@@ -943,8 +967,8 @@ struct e2d_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   e2d::BatchInst *insts = nullptr;   // device
-  e2d::StepCtl *ctls = nullptr;      // device
-  unsigned int *blocks_done = nullptr;
+  e2d::StepCtl *ctls = nullptr;      // device, one per instance
+  int phase = 0;                     // phase of the next launch in the control rings
   int nwaves = 0;
   long long cells = 0;
   hipGraphExec_t graph = nullptr;
@@ -955,7 +979,9 @@ struct e2d_batch {
 
 namespace {
 
-constexpr int kGraphSteps = 32;
+// steps per captured graph: a multiple of 3, so that a replay that starts at
+// phase 0 of the control rings ends there
+constexpr int kGraphSteps = 30;
 
 int batch_check(e2d_batch *b) {
   if (!b) return fail(E2D_ERR_ARG, "null e2d_batch handle");
@@ -999,8 +1025,9 @@ int batch_launch(e2d_batch *b) {
   e2d::KParams k0 = b->runs[0]->kp;
   k0.s.floor_sqrt = b->floor_sqrt;
   for (const e2d_run *r : b->runs) k0.wide_addr |= r->kp.wide_addr;
-  HIP_TRY(e2d::launch_step_batch(k0, b->insts, (int)b->runs.size(), b->ctls, b->nwaves,
-                                 b->seg_rows, b->blocks_done, b->stream));
+  HIP_TRY(e2d::launch_step_batch(k0, b->insts, (int)b->runs.size(), b->ctls, b->phase,
+                                 b->nwaves, b->seg_rows, b->stream));
+  b->phase = b->phase == 2 ? 0 : b->phase + 1;
   return E2D_OK;
 }
 
@@ -1008,15 +1035,21 @@ int batch_graph(e2d_batch *b) {
   if (b->graph && b->graph_tile == b->seg_rows) return E2D_OK;
   batch_drop_graph(b);
   hipGraph_t g = nullptr;
+  // the graph's launches take the phases 0, 1, 2, 0, ... (it is replayed only
+  // at phase 0); capturing runs nothing, so the phase is restored afterwards
+  const int phase = b->phase;
+  b->phase = 0;
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < kGraphSteps; ++k) {
     int rc = batch_launch(b);
     if (rc) {
+      b->phase = phase;
       (void)hipStreamEndCapture(b->stream, &g);
       if (g) (void)hipGraphDestroy(g);
       return rc;
     }
   }
+  b->phase = phase;
   HIP_TRY(hipStreamEndCapture(b->stream, &g));
   hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -1026,13 +1059,22 @@ int batch_graph(e2d_batch *b) {
   return E2D_OK;
 }
 
+// state of instance i after the launched steps
+int batch_state(e2d_batch *b, int i, e2d::StepState &s) {
+  HIP_TRY(hipMemcpyAsync(&s, &b->ctls[i].s[b->phase], sizeof(s), hipMemcpyDeviceToHost,
+                         b->stream));
+  return E2D_OK;
+}
+
 int batch_active(e2d_batch *b, int *active) {
-  std::vector<e2d::StepCtl> c(b->runs.size());
-  HIP_TRY(hipMemcpyAsync(c.data(), b->ctls, c.size() * sizeof(e2d::StepCtl),
-                         hipMemcpyDeviceToHost, b->stream));
+  std::vector<e2d::StepState> s(b->runs.size());
+  for (size_t i = 0; i < s.size(); ++i) {
+    int rc = batch_state(b, (int)i, s[i]);
+    if (rc) return rc;
+  }
   HIP_TRY(hipStreamSynchronize(b->stream));
   int n = 0;
-  for (auto &x : c) n += x.active ? 1 : 0;
+  for (size_t i = 0; i < s.size(); ++i) n += loop_view(b->runs[i]->p, s[i]).active ? 1 : 0;
   *active = n;
   return E2D_OK;
 }
@@ -1068,20 +1110,16 @@ int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_bat
       e2d_batch_destroy(b);
       return hip_fail(e, "hipStreamCreate");
     }
-    void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
+    void *p1 = nullptr, *p2 = nullptr;
     if (hipMalloc(&p1, sizeof(e2d::BatchInst) * count) != hipSuccess ||
-        hipMalloc(&p2, sizeof(e2d::StepCtl) * count) != hipSuccess ||
-        hipMalloc(&p3, 64) != hipSuccess) {
+        hipMalloc(&p2, sizeof(e2d::StepCtl) * count) != hipSuccess) {
       if (p1) (void)hipFree(p1);
-      if (p2) (void)hipFree(p2);
       e2d_batch_destroy(b);
       return fail(E2D_ERR_ALLOC, "batch tables");
     }
     b->insts = static_cast<e2d::BatchInst *>(p1);
     b->ctls = static_cast<e2d::StepCtl *>(p2);
-    b->blocks_done = static_cast<unsigned int *>(p3);
     e = hipMemsetAsync(b->ctls, 0, sizeof(e2d::StepCtl) * count, b->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b->blocks_done, 0, 64, b->stream);
     if (e != hipSuccess) {
       e2d_batch_destroy(b);
       return hip_fail(e, "hipMemsetAsync(batch tables)");
@@ -1105,7 +1143,6 @@ int e2d_batch_destroy(e2d_batch *b) {
   for (e2d_run *r : b->runs) e2d_destroy(r);
   if (b->insts) (void)hipFree(b->insts);
   if (b->ctls) (void)hipFree(b->ctls);
-  if (b->blocks_done) (void)hipFree(b->blocks_done);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return E2D_OK;
@@ -1114,8 +1151,9 @@ int e2d_batch_destroy(e2d_batch *b) {
 int e2d_batch_init(e2d_batch *b) {
   int rc = batch_check(b);
   if (rc) return rc;
+  // phase 0 of every instance's rings: t = 0, nStep = 0, the rest cleared
+  b->phase = 0;
   HIP_TRY(hipMemsetAsync(b->ctls, 0, sizeof(e2d::StepCtl) * b->runs.size(), b->stream));
-  HIP_TRY(hipMemsetAsync(b->blocks_done, 0, 64, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   for (size_t i = 0; i < b->runs.size(); ++i) {
     e2d_run *r = b->runs[i];
@@ -1123,8 +1161,7 @@ int e2d_batch_init(e2d_batch *b) {
     if (rc) return rc;
     HIP_TRY(e2d::launch_boundaries(r->kp, r->U, r->stream));
     HIP_TRY(e2d::launch_boundaries(r->kp, r->U2, r->stream));
-    HIP_TRY(e2d::launch_dt_reduce(r->kp, r->U, &b->ctls[i].invdt, r->stream));
-    HIP_TRY(e2d::launch_dt_finalize(r->kp, b->ctls + i, r->stream));
+    HIP_TRY(e2d::launch_dt_reduce(r->kp, r->U, &b->ctls[i].rate[0][0], r->stream));
   }
   for (e2d_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
   return E2D_OK;
@@ -1153,7 +1190,7 @@ int e2d_batch_run(e2d_batch *b, int max_steps, int use_graph, double *elapsed_ms
   HIP_TRY(hipEventRecord(e0, b->stream));
   int done = 0;
   while (done < max_steps) {
-    if (use_graph && max_steps - done >= b->graph_steps) {
+    if (use_graph && b->phase == 0 && max_steps - done >= b->graph_steps) {
       HIP_TRY(hipGraphLaunch(b->graph, b->stream));
       done += b->graph_steps;
     } else {
@@ -1180,11 +1217,12 @@ int e2d_batch_status(e2d_batch *b, int instance, double *t, int *nStep) {
   int rc = batch_check(b);
   if (rc) return rc;
   if (instance < 0 || instance >= (int)b->runs.size()) return fail(E2D_ERR_ARG, "bad instance");
-  e2d::StepCtl c{};
-  HIP_TRY(hipMemcpyAsync(&c, b->ctls + instance, sizeof(c), hipMemcpyDeviceToHost, b->stream));
+  e2d::StepState s{};
+  rc = batch_state(b, instance, s);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
-  if (t) *t = c.t;
-  if (nStep) *nStep = c.nstep;
+  if (t) *t = s.t;
+  if (nStep) *nStep = s.nstep;
   return E2D_OK;
 }
 

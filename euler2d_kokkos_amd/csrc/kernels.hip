@@ -507,19 +507,6 @@ __global__ __launch_bounds__(256) void dt_reduce_kernel(KParams kp, Grid U,
   }
 }
 
-// Starts a run loop: dt of the pending step from the reduced invDt, with the
-// tEnd clamp and the while condition of main.cpp:100,128-134.
-__global__ void dt_finalize_kernel(KParams kp, StepCtl *ctl) {
-  const double invdt = __longlong_as_double((long long)ctl->invdt);
-  const double t = ctl->t;
-  double dt = kp.cfl / invdt;
-  if (t + dt > kp.tEnd) dt = kp.tEnd - t;
-  ctl->dt = dt;
-  ctl->active = (t < kp.tEnd && ctl->nstep < kp.nStepmax) ? 1 : 0;
-  ctl->invdt = 0ull;
-  ctl->blocks_done = 0u;
-}
-
 // ------------------------------------------------ reference pass schedule
 __global__ void prim_kernel(KParams kp, Grid U, Grid Q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -705,9 +692,10 @@ __global__ void fluxes_atomic_kernel(KParams kp, Grid Q, Grid U, double dtdx, do
 // Ghost cells are produced on the fly from the closed-form BC map; the wave
 // owning a ghost cell also writes it to `in` and `out`, as make_boundaries(in)
 // followed by deep_copy(out, in) leaves them in the reference.
-// In run-loop mode (ctl != nullptr) the kernel reads dt from the control
-// block, folds the next step's CFL rate of every updated cell into a wave max,
-// and the last block to finish advances t, nStep and the next dt.
+// In run-loop mode (ctl != nullptr) every wavefront derives t, dt and the
+// while condition from the control rings, one wavefront publishes the state
+// after the step, and the CFL rate of every updated cell is folded into a
+// wave max for the next launch's dt.
 
 // Whole-wavefront lane shifts by one with DPP (GFX9 wave_shr:1 / wave_shl:1):
 // a full-rate VALU move instead of an LDS-routed permute.  Lane 0 (shift up)
@@ -958,39 +946,69 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
   return m.wmax;
 }
 
-// Loop-state advance of one run (main.cpp:128-143): t += dt, nStep++, the
-// next dt from the reduced invDt with the tEnd clamp, and the while condition.
-__device__ __forceinline__ void advance_ctl(const KParams &kp, StepCtl *ctl) {
-  const unsigned long long bits = atomicMax(&ctl->invdt, 0ull);
-  const double invdt = __longlong_as_double((long long)bits);
-  const double t = ctl->t + ctl->dt;  // t += dt (main.cpp:143)
-  const int n = ctl->nstep + 1;
-  double dtn = kp.cfl / invdt;       // HydroRun.h:246
-  if (t + dtn > kp.tEnd) dtn = kp.tEnd - t;  // main.cpp:131-134
-  ctl->t = t;
-  ctl->nstep = n;
-  ctl->dt = dtn;
-  ctl->active = (t < kp.tEnd && n < kp.nStepmax) ? 1 : 0;
-  atomicExch(&ctl->invdt, 0ull);
+// Position of launch phase ph of a run loop (main.cpp:100,128-134): the time
+// and step count before the step, the while condition, and dt = cfl / invDt
+// (HydroRun.h:246) from the CFL rate of the step's input, clamped to tEnd.
+// Every wavefront of the launch derives the same values from the same words.
+struct LoopPos {
+  double t, dt;
+  int nstep;
+  bool active;
+};
+
+__device__ __forceinline__ LoopPos loop_pos(const KParams &kp, const StepCtl *ctl, int ph) {
+  LoopPos L;
+  L.t = ctl->s[ph].t;
+  L.nstep = ctl->s[ph].nstep;
+  L.active = L.t < kp.tEnd && L.nstep < kp.nStepmax;
+  unsigned long long m = 0ull;  // max of positive doubles = max of their bits
+  for (int k = 0; k < kRateSlots; ++k) {
+    const unsigned long long b = ctl->rate[ph][k * kSlotStride];
+    m = b > m ? b : m;
+  }
+  double dt = kp.cfl / __longlong_as_double((long long)m);
+  if (L.t + dt > kp.tEnd) dt = kp.tEnd - L.t;
+  L.dt = dt;
+  return L;
 }
 
-// Folds the wave max into *invdt and reports whether this block is the last
-// of the grid to finish (all blocks' atomics are then visible).
-// The wave maxima of a block are merged per target (in the batched kernel the
-// waves of one block can belong to different instances), so that a block
-// issues one device-scope atomic per distinct target instead of one per wave.
+// The step's one writer, the lanes of one wavefront: the state after the step
+// (main.cpp:143, t += dt and nStep++; unchanged once the loop has ended) and
+// the cleared rate entry of the launch after the next.  Nothing in this launch
+// reads either.
+__device__ __forceinline__ void loop_publish(StepCtl *ctl, int ph, const LoopPos &L, int lane) {
+  const int p1 = ph == 2 ? 0 : ph + 1;
+  const int p2 = ph == 0 ? 2 : ph - 1;
+  if (lane < kRateSlots) ctl->rate[p2][lane * kSlotStride] = 0ull;
+  if (lane == 0) {
+    StepState n;
+    n.t = L.active ? L.t + L.dt : L.t;
+    n.nstep = L.active ? L.nstep + 1 : L.nstep;
+    n.pad = 0;
+    ctl->s[p1] = n;
+  }
+}
+
+// The rate slot block b of launch phase ph folds into.
+__device__ __forceinline__ unsigned long long *rate_slot(StepCtl *ctl, int ph) {
+  const int p1 = ph == 2 ? 0 : ph + 1;
+  return &ctl->rate[p1][(blockIdx.x % kRateSlots) * kSlotStride];
+}
+
+// Folds the wave maxima of a block into their targets: one device-scope
+// atomic per block and distinct target (the max is exact and order
+// independent).  In the batched kernel the waves of one block can belong to
+// different instances, hence the merge per target.
 template <class T>
-__device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long long *invdt,
-                                                       unsigned int *blocks_done) {
+__device__ __forceinline__ void reduce_rate_block(T wmax, unsigned long long *target) {
   constexpr int kWaves = kBlock / kLanes;
   __shared__ double wm[kWaves];
   __shared__ unsigned long long *wp[kWaves];
-  __shared__ bool last;
   const double m = (double)wave_max(wmax);
   if ((threadIdx.x & (kLanes - 1)) == 0) {
     const int w = threadIdx.x / kLanes;
     wm[w] = m;
-    wp[w] = invdt;
+    wp[w] = target;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1005,15 +1023,7 @@ __device__ __forceinline__ bool reduce_rate_last_block(T wmax, unsigned long lon
         }
       if (b > 0.0) atomicMax(p, dbits(b));
     }
-    // release before counting the block done, acquire only in the last block:
-    // a full fence here would also invalidate the XCD's L2 in every block
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned int prev = atomicAdd(blocks_done, 1u);
-    last = (prev == gridDim.x - 1);
-    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
-  __syncthreads();
-  return last;
 }
 
 // True when every column the wavefront of strip `strip` loads (60 strip ..
@@ -1045,10 +1055,16 @@ __device__ __forceinline__ void fused_wave(const FusedMap &f, int wave, int &str
 // O32: 32-bit row offsets (every plane at most 4 GiB).
 template <int RS, class P, bool LOOP, bool O32>
 __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in, Grid out,
-                                                            double dt_arg, StepCtl *ctl,
+                                                            double dt_arg, StepCtl *ctl, int ph,
                                                             FusedMap fm) {
   using T = typename P::T;
-  if (LOOP && !ctl->active) return;  // loop finished: nothing to do (uniform)
+  double dt = dt_arg;
+  if constexpr (LOOP) {
+    const LoopPos L = loop_pos(kp, ctl, ph);
+    if (blockIdx.x == 0 && threadIdx.x < kLanes) loop_publish(ctl, ph, L, threadIdx.x);
+    if (!L.active) return;  // loop finished: nothing to do (uniform)
+    dt = L.dt;
+  }
   // the wave index is uniform: every row index and the row ghost map derived
   // from it are then scalar values and scalar control flow
   const int wave =
@@ -1057,19 +1073,17 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
   if (wave < fm.nwaves) {
     int strip, seg, tile;
     fused_wave(fm, wave, strip, seg, tile);
-    const double dt = LOOP ? ctl->dt : dt_arg;
+    // in and out share their geometry (checked at launch): one pitch and one
+    // plane stride let a row's offset serve both arrays
+    Grid o = in;
+    o.p = out.p;
     if (interior_cols(kp, strip))  // uniform
-      wmax = march<RS, P, LOOP, O32, false>(kp, in, out, dt, tile, fm.nstrips, strip, seg);
+      wmax = march<RS, P, LOOP, O32, false>(kp, in, o, dt, tile, fm.nstrips, strip, seg);
     else
-      wmax = march<RS, P, LOOP, O32, true>(kp, in, out, dt, tile, fm.nstrips, strip, seg);
+      wmax = march<RS, P, LOOP, O32, true>(kp, in, o, dt, tile, fm.nstrips, strip, seg);
   }
-  if (!LOOP) return;
-  // run-loop epilogue: next-step invDt (exact max, order independent) and the
-  // last block advances the loop state
-  if (reduce_rate_last_block(wmax, &ctl->invdt, &ctl->blocks_done) && threadIdx.x == 0) {
-    advance_ctl(kp, ctl);
-    atomicExch(&ctl->blocks_done, 0u);
-  }
+  // run loop: the CFL rate of the step's output, for the next launch's dt
+  if constexpr (LOOP) reduce_rate_block(wmax, rate_slot(ctl, ph));
 }
 
 // ------------------------------------------------------- batched step kernel
@@ -1077,19 +1091,17 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
 // One launch advances every active instance of a batch by one step, each with
 // its own dt.  Wavefront w works on the instance whose wave range contains w
 // (binary search over wave_begin), on that instance's strip / segment; its
-// in/out arrays follow the instance's own step parity.  The last block of the
-// grid advances every instance's loop state.
+// in/out arrays follow the instance's own step parity.  Each instance has its
+// own loop control rings (StepCtl), published by its first wavefront.
 template <int RS, class P, bool O32>
 __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__restrict__ insts,
-                                                            int ninst,
-                                                            StepCtl *ctls, int nwaves,
-                                                            int seg_rows,
-                                                            unsigned int *blocks_done) {
+                                                            int ninst, StepCtl *ctls, int ph,
+                                                            int nwaves, int seg_rows) {
   using T = typename P::T;
   const int wave =
       __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6));
   T wmax = T(0.0);
-  int inst = -1;
+  unsigned long long *target = nullptr;
   if (wave < nwaves) {
     int lo = 0, hi = ninst - 1;
     while (lo < hi) {  // last instance with wave_begin <= wave
@@ -1099,33 +1111,32 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__r
       else
         hi = mid - 1;
     }
-    inst = __builtin_amdgcn_readfirstlane(lo);  // wave-uniform: scalar loads of the table
+    const int inst = __builtin_amdgcn_readfirstlane(lo);  // wave-uniform: scalar loads of the table
     const BatchInst &bi = insts[inst];
     StepCtl *ctl = ctls + inst;
-    if (ctl->active) {
-      const int w = wave - bi.wave_begin;
-      const bool even = (ctl->nstep & 1) == 0;
-      // by-value copies: the march then reads its parameters from scalar
-      // registers instead of reloading them around every store
-      const KParams kp = bi.kp;
+    // by-value copies: the march then reads its parameters from scalar
+    // registers instead of reloading them around every store
+    const KParams kp = bi.kp;
+    const LoopPos L = loop_pos(kp, ctl, ph);
+    const int w = wave - bi.wave_begin;
+    if (w == 0) loop_publish(ctl, ph, L, threadIdx.x & (kLanes - 1));
+    if (L.active) {
+      const bool even = (L.nstep & 1) == 0;
+      // U and U2 share their geometry (one allocation shape per instance)
       const Grid gin = even ? bi.U : bi.U2;
-      const Grid gout = even ? bi.U2 : bi.U;
+      Grid gout = gin;
+      gout.p = even ? bi.U2.p : bi.U.p;
       const int strip = w % bi.nstrips, seg = w / bi.nstrips;
       if (interior_cols(kp, strip))
-        wmax = march<RS, P, true, O32, false>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips,
+        wmax = march<RS, P, true, O32, false>(kp, gin, gout, L.dt, seg_rows, bi.nstrips,
                                               strip, seg);
       else
-        wmax = march<RS, P, true, O32, true>(kp, gin, gout, ctl->dt, seg_rows, bi.nstrips,
+        wmax = march<RS, P, true, O32, true>(kp, gin, gout, L.dt, seg_rows, bi.nstrips,
                                              strip, seg);
+      target = rate_slot(ctl, ph);
     }
   }
-  const bool last = reduce_rate_last_block(wmax, inst >= 0 ? &ctls[inst].invdt : nullptr,
-                                           blocks_done);
-  if (last) {
-    for (int i = threadIdx.x; i < ninst; i += blockDim.x)
-      if (ctls[i].active) advance_ctl(insts[i].kp, ctls + i);
-    if (threadIdx.x == 0) atomicExch(blocks_done, 0u);
-  }
+  reduce_rate_block(wmax, target);
 }
 
 // ----------------------------------------------------------- radial profile
@@ -1236,40 +1247,40 @@ dim3 grid2d(int nx, int ny, dim3 blk) {
 }
 
 template <int RS, class P>
-void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl, int seg_rows,
-                    hipStream_t st) {
+void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl, int ph,
+                    int seg_rows, hipStream_t st) {
   const FusedMap fm = fused_map(kp, seg_rows);
   const int waves_per_block = kBlock / kLanes;
   const int blocks = (fm.nwaves + waves_per_block - 1) / waves_per_block;
   if (ctl && off32_ok(kp, in) && off32_ok(kp, out))
     hipLaunchKernelGGL((step_fused_kernel<RS, P, true, true>), dim3(blocks), dim3(kBlock), 0, st,
-                       kp, in, out, dt, ctl, fm);
+                       kp, in, out, dt, ctl, ph, fm);
   else if (ctl)
     hipLaunchKernelGGL((step_fused_kernel<RS, P, true, false>), dim3(blocks), dim3(kBlock), 0, st,
-                       kp, in, out, dt, ctl, fm);
+                       kp, in, out, dt, ctl, ph, fm);
   else
     hipLaunchKernelGGL((step_fused_kernel<RS, P, false, false>), dim3(blocks), dim3(kBlock), 0,
-                       st, kp, in, out, dt, ctl, fm);
+                       st, kp, in, out, dt, ctl, ph, fm);
 }
 
 template <class P>
-void launch_fused_p(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl, int seg_rows,
-                    hipStream_t st) {
+void launch_fused_p(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl, int ph,
+                    int seg_rows, hipStream_t st) {
   if constexpr (std::is_same<P, P64>::value) {
     if (kp.s.floor_sqrt && kp.s.riemann != 0) {  // HLL / HLLC with the short square root
       if (kp.s.riemann == 2)
-        launch_fused_t<2 | kRiemannFloorSqrt, P>(kp, in, out, dt, ctl, seg_rows, st);
+        launch_fused_t<2 | kRiemannFloorSqrt, P>(kp, in, out, dt, ctl, ph, seg_rows, st);
       else
-        launch_fused_t<1 | kRiemannFloorSqrt, P>(kp, in, out, dt, ctl, seg_rows, st);
+        launch_fused_t<1 | kRiemannFloorSqrt, P>(kp, in, out, dt, ctl, ph, seg_rows, st);
       return;
     }
   }
   if (kp.s.riemann == 2)
-    launch_fused_t<2, P>(kp, in, out, dt, ctl, seg_rows, st);
+    launch_fused_t<2, P>(kp, in, out, dt, ctl, ph, seg_rows, st);
   else if (kp.s.riemann == 1)
-    launch_fused_t<1, P>(kp, in, out, dt, ctl, seg_rows, st);
+    launch_fused_t<1, P>(kp, in, out, dt, ctl, ph, seg_rows, st);
   else
-    launch_fused_t<0, P>(kp, in, out, dt, ctl, seg_rows, st);
+    launch_fused_t<0, P>(kp, in, out, dt, ctl, ph, seg_rows, st);
 }
 
 }  // namespace
@@ -1368,11 +1379,6 @@ hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt
   return hipGetLastError();
 }
 
-hipError_t launch_dt_finalize(const KParams &kp, StepCtl *ctl, hipStream_t st) {
-  hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, st, kp, ctl);
-  return hipGetLastError();
-}
-
 hipError_t launch_prim(const KParams &kp, Grid U, Grid Q, hipStream_t st) {
   const dim3 blk(64, 4);
   hipLaunchKernelGGL(prim_kernel, grid2d(kp.isize, kp.jsize, blk), blk, 0, st, kp, U, Q);
@@ -1447,14 +1453,16 @@ hipError_t launch_fluxes_atomic(const KParams &kp, Grid Q, Grid U, double dtdx, 
 }
 
 hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl,
-                             int seg_rows, hipStream_t st) {
+                             int ph, int seg_rows, hipStream_t st) {
+  if (ctl && (ph < 0 || ph > 2)) return hipErrorInvalidValue;
+  if (in.pitch != out.pitch || in.plane != out.plane) return hipErrorInvalidValue;
   if (seg_rows < 1) seg_rows = 1;
   if (kp.prec == E2D_PRECISION_FP32_SPLIT)
-    launch_fused_p<PSplit>(kp, in, out, dt, ctl, seg_rows, st);
+    launch_fused_p<PSplit>(kp, in, out, dt, ctl, ph, seg_rows, st);
   else if (kp.prec == E2D_PRECISION_FP32)
-    launch_fused_p<P32>(kp, in, out, dt, ctl, seg_rows, st);
+    launch_fused_p<P32>(kp, in, out, dt, ctl, ph, seg_rows, st);
   else
-    launch_fused_p<P64>(kp, in, out, dt, ctl, seg_rows, st);
+    launch_fused_p<P64>(kp, in, out, dt, ctl, ph, seg_rows, st);
   return hipGetLastError();
 }
 
@@ -1478,17 +1486,17 @@ int batch_waves(const KParams &kp, int seg_rows, int *nstrips) {
 }
 
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
-                             StepCtl *ctls_dev, int nwaves, int seg_rows,
-                             unsigned int *blocks_done, hipStream_t st) {
+                             StepCtl *ctls_dev, int ph, int nwaves, int seg_rows, hipStream_t st) {
+  if (ph < 0 || ph > 2) return hipErrorInvalidValue;
   const int blocks = (nwaves + kBlock / kLanes - 1) / (kBlock / kLanes);
 #define E2D_BATCH_LAUNCH(RS, P)                                                            \
   do {                                                                                     \
     if (kp0.wide_addr)                                                                     \
       hipLaunchKernelGGL((step_batch_kernel<RS, P, false>), dim3(blocks), dim3(kBlock), 0,   \
-                         st, insts_dev, ninst, ctls_dev, nwaves, seg_rows, blocks_done);   \
+                         st, insts_dev, ninst, ctls_dev, ph, nwaves, seg_rows);               \
     else                                                                                   \
       hipLaunchKernelGGL((step_batch_kernel<RS, P, true>), dim3(blocks), dim3(kBlock), 0,    \
-                         st, insts_dev, ninst, ctls_dev, nwaves, seg_rows, blocks_done);   \
+                         st, insts_dev, ninst, ctls_dev, ph, nwaves, seg_rows);               \
   } while (0)
   if (kp0.prec == E2D_PRECISION_FP32_SPLIT) {
     if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, PSplit);

@@ -43,16 +43,26 @@ struct KParams {
   int wide_addr;
 };
 
-// Run-loop control block kept in device memory (time loop of main.cpp:100-145
-// without a host round trip per step).
-struct StepCtl {
-  double t;                     // time at the start of the pending step
-  double dt;                    // dt of the pending step (already clamped to tEnd)
-  unsigned long long invdt;     // max of (c+|u|)/dx+(c+|v|)/dy as IEEE bits
-  int nstep;                    // steps executed so far
-  int active;                   // pending step runs iff t < tEnd && nstep < nStepmax
-  unsigned int blocks_done;     // last-block counter of the fused step kernel
+// Run-loop control block kept in device memory: the time loop of
+// main.cpp:100-145 without a host round trip per step.  Launch L of a loop
+// (phase ph = L % 3) reads s[ph], the time and step count before its step, and
+// rate[ph], the CFL rate of its input, from which every wavefront derives the
+// same dt (HydroRun.h:246, with the tEnd clamp) and while condition.  One
+// wavefront writes s[(ph + 1) % 3] and clears rate[(ph + 2) % 3]; the blocks
+// fold the rate of the step's output into rate[(ph + 1) % 3].  No launch reads
+// what it writes, so no block waits for another at the end of a step.
+struct StepState {
+  double t;   // time before the step
+  int nstep;  // steps executed before it
   int pad;
+};
+constexpr int kRateSlots = 16;   // block b folds into slot b % kRateSlots
+constexpr int kSlotStride = 16;  // one 128-byte line per slot
+struct StepCtl {
+  StepState s[3];
+  unsigned long long pad[10];
+  // max over the slots of (c+|u|)/dx+(c+|v|)/dy as IEEE bits (0 = none yet)
+  unsigned long long rate[3][kRateSlots * kSlotStride];
 };
 
 // One instance of a batch as the batched step kernel sees it.
@@ -103,7 +113,6 @@ hipError_t launch_blast_energy(const KParams &kp, const e2d_params &p, Grid U, d
 hipError_t launch_boundaries(const KParams &kp, Grid U, hipStream_t st);
 hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt_bits,
                             hipStream_t st);
-hipError_t launch_dt_finalize(const KParams &kp, StepCtl *ctl, hipStream_t st);
 // reference pass schedule (FP64 only)
 hipError_t launch_prim(const KParams &kp, Grid U, Grid Q, hipStream_t st);
 hipError_t launch_fluxes(const KParams &kp, Grid Q, Grid Fx, Grid Fy, double dtdx, double dtdy,
@@ -118,9 +127,10 @@ hipError_t launch_update_dir(const KParams &kp, int dir, Grid U, Grid F, hipStre
 // implementation version 2: fluxes added to both neighbours with FP64 atomics
 hipError_t launch_fluxes_atomic(const KParams &kp, Grid Q, Grid U, double dtdx, double dtdy,
                                 hipStream_t st);
-// ctl == nullptr: explicit dt (HydroRun::godunov_unsplit); else run-loop mode
+// ctl == nullptr: explicit dt (HydroRun::godunov_unsplit); else run-loop mode,
+// launch phase ph (0..2) of the loop's control rings
 hipError_t launch_step_fused(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ctl,
-                             int seg_rows, hipStream_t st);
+                             int ph, int seg_rows, hipStream_t st);
 hipError_t launch_sums(const KParams &kp, Grid U, double *partials, double *out4, hipStream_t st);
 int sums_partials_needed(const KParams &kp);
 // wavefronts the fused step launches for one grid (and its strip count)
@@ -129,8 +139,7 @@ int batch_waves(const KParams &kp, int seg_rows, int *nstrips);
 // (kp0.s.floor_sqrt must hold for every instance; kp0.wide_addr must be set
 // when any instance has a plane above 4 GiB)
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
-                             StepCtl *ctls_dev, int nwaves, int seg_rows,
-                             unsigned int *blocks_done, hipStream_t st);
+                             StepCtl *ctls_dev, int ph, int nwaves, int seg_rows, hipStream_t st);
 // per-bin cell counts and density sums of the radial profile (nbins entries each)
 hipError_t launch_radial_profile(const KParams &kp, Grid U, const RadialGeom &geo,
                                  unsigned long long *counts, double *sums, hipStream_t st);
