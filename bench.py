@@ -44,21 +44,23 @@ DECKS = {  # deck, description, steps of the workload
 }
 
 
-PROFILE = ROOT / "profiles" / "step_fused_fp64_blast_1024x1536.md"
+PROFILES = ROOT / "profiles"
 
 
 def profiled_traffic(workload: str, tile: int):
-    """HBM bytes per launch of the step kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary, when it covers this workload and tile."""
+    """HBM bytes per launch of the FP64 step kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summaries (one per tile), when one covers this
+    workload and tile."""
     import re
-    if workload != "blast" or not PROFILE.exists():
+    if workload != "blast":
         return None
-    text = PROFILE.read_text()
-    m = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
-    t = re.search(r"seg_rows (\d+)", text)
-    if not m or not t or int(t.group(1)) != tile:
-        return None
-    return int(m.group(1))
+    for path in sorted(PROFILES.glob("step_fused_fp64_blast_1024x1536*.md")):
+        text = path.read_text()
+        m = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
+        t = re.search(r"seg_rows (\d+)", text)
+        if m and t and int(t.group(1)) == tile:
+            return int(m.group(1))
+    return None
 
 
 def parse_args():
@@ -96,7 +98,7 @@ def batch_line(steps: int) -> dict:
 
     ps = sweep_params(steps=steps)
     b = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
-    tile, _ = b.autotune(candidates=(16, 24, 32, 48, 64), steps=5)
+    tile, _ = b.autotune(candidates=(8, 12, 16, 24, 32, 48, 64), steps=5)
     b.init()
     info = b.info()
     t0 = time.perf_counter()
