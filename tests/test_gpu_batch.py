@@ -47,6 +47,33 @@ def test_batch_equals_single_runs(gpu_available, precision, use_graph):
                                       err_msg=f"instance {i}")
 
 
+def test_batch_graph_replays_from_any_phase(gpu_available):
+    """Runs of uneven lengths, with and without the captured graph, leave the
+    batch where single runs end.  The loop control cycles through three ring
+    entries and the graph replays only from the first one, so each call below
+    starts the graph from a different launch count (7, 38, then 103)."""
+    ps = mixed_instances()
+    for p in ps:  # every instance still active past the graph replays
+        p.nStepmax = 150
+        p.tEnd = 1e9
+    batch = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
+    batch.set_tile(12)
+    steps = 0
+    for k, g in ((7, False), (31, True), (65, True), (2, True)):
+        batch.run(k, use_graph=g)
+        steps += k
+        for i in (0, 3):
+            assert batch.status(i)[1] == steps, (i, steps)
+    for i, p in enumerate(ps):
+        q = p.copy()
+        q.nStepmax = steps
+        run = HydroRun(HydroParams(q), precision=E2D_PRECISION_FP32_SPLIT)
+        run.run()
+        assert batch.status(i) == (run.t, run.nStep), i
+        np.testing.assert_array_equal(batch.state(i), run.get_state(run.current()),
+                                      err_msg=f"instance {i}")
+
+
 @pytest.mark.parametrize("precision", [E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT],
                          ids=["fp64", "split"])
 def test_batch_wide_addressing(gpu_available, precision, monkeypatch):

@@ -205,6 +205,23 @@ def test_conserved_sums_reflecting(gpu_available):
     assert abs(s1[1] - s0[1]) <= 1e-12 * abs(s0[1])
 
 
+def test_loop_in_pieces_matches_oracle(gpu_available):
+    """Run-loop calls of uneven lengths continue one loop (each call restarts
+    the device-side loop control from the state the last one left), and the
+    launches queued past tEnd change nothing: the last call asks for far more
+    steps than remain."""
+    p = _params("tend_clamp")
+    ora = O.OracleRun(p)
+    ora.run()
+    run = HydroRun(HydroParams(p.copy()))
+    for k in (2, 3, 1):
+        run.run(k)
+    assert run.nStep == 6 and ora.nStep == 9
+    run.run(40)
+    _compare(run, ora, "pieces")
+    assert run.t == p.tEnd
+
+
 def test_autotune_restores_state(gpu_available):
     p = _params("blast48x72")
     run = HydroRun(HydroParams(p.copy()))
