@@ -592,6 +592,9 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
     else if (precision == E2D_PRECISION_FP64 && p->implementationVersion == 2)
       r->schedule = E2D_SCHEDULE_ATOMIC;
     HIP_TRY(hipGetDevice(&r->device));
+    int ncu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, r->device));
+    r->kp.simds = 4 * ncu;
     HIP_TRY(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     void *ptr = nullptr;
     int rc2 = alloc_grid(r, r->U, precision);

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <type_traits>
 
@@ -864,10 +865,22 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
 // slopes of row j_base-1; k = 3 the first y and x faces; k = 4 .. seg_rows+2
 // is the steady state (all stages, branch-free); k = seg_rows+3 finishes the
 // last y face and update.
+// Issue priority 3 - lvl (s_setprio takes an immediate).
+__device__ __forceinline__ void set_prio_level(int lvl) {
+  if (lvl <= 0)
+    __builtin_amdgcn_s_setprio(3);
+  else if (lvl == 1)
+    __builtin_amdgcn_s_setprio(2);
+  else if (lvl == 2)
+    __builtin_amdgcn_s_setprio(1);
+  else
+    __builtin_amdgcn_s_setprio(0);
+}
+
 template <int RS, class P, bool RATE, bool O32, bool COLS>
 __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in, const Grid &out,
                                                double dt, int seg_rows, int nstrips, int strip,
-                                               int seg) {
+                                               int seg, bool prio) {
   using T = typename P::T;
   const int lane = threadIdx.x & (kLanes - 1);
   const T dtdx = (T)(dt / kp.dx);
@@ -934,7 +947,18 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
   // copied.  Only the top segment has iterations past that bound.
   const int k_rows = min(rows + 2, kp.ny - r0);
   int k = 4;
+  // prio: the issue priority falls from 3 to 0 over the steady rows, so that
+  // of two wavefronts sharing a SIMD the one behind wins arbitration
+  // (level floor(4 k / (k_rows + 1)), set when k reaches the next level)
+  const int n1 = k_rows + 1;
+  int k_next = prio ? k : INT_MAX;
   for (; k + 2 <= k_rows; k += 3) {
+    if (k >= k_next) {
+      const int k4 = 4 * k;
+      const int lvl = (k4 >= n1) + (k4 >= 2 * n1) + (k4 >= 3 * n1);
+      set_prio_level(lvl);
+      k_next = lvl < 3 ? ((lvl + 1) * n1 + 3) >> 2 : INT_MAX;
+    }
     E2D_ROW(kAll, true, RATE, false, k);
     E2D_ROW(kAll, true, RATE, false, k + 1);
     E2D_ROW(kAll, true, RATE, false, k + 2);
@@ -1078,9 +1102,11 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
     Grid o = in;
     o.p = out.p;
     if (interior_cols(kp, strip))  // uniform
-      wmax = march<RS, P, LOOP, O32, false>(kp, in, o, dt, tile, fm.nstrips, strip, seg);
+      wmax = march<RS, P, LOOP, O32, false>(kp, in, o, dt, tile, fm.nstrips, strip, seg,
+                                            fm.prio != 0);
     else
-      wmax = march<RS, P, LOOP, O32, true>(kp, in, o, dt, tile, fm.nstrips, strip, seg);
+      wmax = march<RS, P, LOOP, O32, true>(kp, in, o, dt, tile, fm.nstrips, strip, seg,
+                                           fm.prio != 0);
   }
   // run loop: the CFL rate of the step's output, for the next launch's dt
   if constexpr (LOOP) reduce_rate_block(wmax, rate_slot(ctl, ph));
@@ -1129,10 +1155,10 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__r
       const int strip = w % bi.nstrips, seg = w / bi.nstrips;
       if (interior_cols(kp, strip))
         wmax = march<RS, P, true, O32, false>(kp, gin, gout, L.dt, seg_rows, bi.nstrips,
-                                              strip, seg);
+                                              strip, seg, false);
       else
         wmax = march<RS, P, true, O32, true>(kp, gin, gout, L.dt, seg_rows, bi.nstrips,
-                                             strip, seg);
+                                             strip, seg, false);
       target = rate_slot(ctl, ph);
     }
   }
@@ -1476,6 +1502,7 @@ FusedMap fused_map(const KParams &kp, int seg_rows) {
   const int nsegs_e = (kp.ny + f.tile_e - 1) / f.tile_e;
   f.n_int = (f.nstrips - f.nedge) * nsegs_i;
   f.nwaves = f.n_int + f.nedge * nsegs_e;
+  f.prio = kp.simds > 0 && f.nwaves > kp.simds && f.nwaves <= 2 * kp.simds ? 1 : 0;
   return f;
 }
 

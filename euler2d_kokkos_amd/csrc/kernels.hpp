@@ -41,6 +41,8 @@ struct KParams {
   // arithmetic even where every plane is at most 4 GiB (set for planes above
   // 4 GiB, or through E2D_WIDE_ADDRESSING=1 to exercise that form)
   int wide_addr;
+  // SIMDs of the device (4 per CU), for the fused step's issue-priority rule
+  int simds;
 };
 
 // Run-loop control block kept in device memory: the time loop of
@@ -97,6 +99,10 @@ struct FusedMap {
   int nstrips, nedge;  // strips of 60 columns; edge strips (min(nstrips, 2))
   int n_int;           // wavefronts of the interior strips
   int nwaves;          // all wavefronts
+  // 1 when every wavefront is resident at once, two per SIMD: a wavefront then
+  // lowers its issue priority as its march progresses (s_setprio), so that the
+  // two on a SIMD finish together instead of the younger running alone last
+  int prio;
 };
 FusedMap fused_map(const KParams &kp, int seg_rows);
 
