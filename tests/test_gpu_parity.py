@@ -131,6 +131,22 @@ def test_interior_wavefronts_match_oracle(gpu_available, tile, bc):
     _compare(run, ora, f"interior/{tile}/{bc}")
 
 
+def test_two_wavefronts_per_simd_matches_oracle(gpu_available):
+    """The blast workload's own grid, 1024 x 1536, at the 14-row tile: 2016
+    wavefronts, which on a 256-CU MI355X are resident two per SIMD at once, so
+    each march also steps its issue priority down as it progresses."""
+    p = O.default_params(nx=1024, ny=1536, ymax=1.5, problemType=1, cfl=0.7, nStepmax=6,
+                         tEnd=1e9, blast_radius=0.1, blast_center_x=0.5, blast_center_y=0.75,
+                         blast_density_in=1.0, blast_density_out=1.0, blast_pressure_in=10.0,
+                         blast_pressure_out=0.1)
+    ora = O.OracleRun(p)
+    ora.run()
+    run = HydroRun(HydroParams(p.copy()))
+    run.set_tile(14)
+    run.run()
+    _compare(run, ora, "blast1024x1536/14")
+
+
 def test_planes_above_4gib(gpu_available):
     """1000 x 540000 cells: each FP64 plane holds 4.35 GB, so the step kernel
     addresses rows with 64-bit offsets; the blast sits at rows ~539000, beyond
