@@ -972,6 +972,7 @@ struct e2d_batch {
   e2d::BatchInst *insts = nullptr;   // device
   e2d::StepCtl *ctls = nullptr;      // device, one per instance
   int phase = 0;                     // phase of the next launch in the control rings
+  bool ready = false;                // e2d_batch_init has set every instance's loop state
   int nwaves = 0;
   long long cells = 0;
   hipGraphExec_t graph = nullptr;
@@ -1155,6 +1156,7 @@ int e2d_batch_init(e2d_batch *b) {
   int rc = batch_check(b);
   if (rc) return rc;
   // phase 0 of every instance's rings: t = 0, nStep = 0, the rest cleared
+  b->ready = false;
   b->phase = 0;
   HIP_TRY(hipMemsetAsync(b->ctls, 0, sizeof(e2d::StepCtl) * b->runs.size(), b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
@@ -1167,6 +1169,7 @@ int e2d_batch_init(e2d_batch *b) {
     HIP_TRY(e2d::launch_dt_reduce(r->kp, r->U, &b->ctls[i].rate[0][0], r->stream));
   }
   for (e2d_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
+  b->ready = true;
   return E2D_OK;
 }
 
@@ -1183,6 +1186,7 @@ int e2d_batch_run(e2d_batch *b, int max_steps, int use_graph, double *elapsed_ms
   int rc = batch_check(b);
   if (rc) return rc;
   if (max_steps <= 0) return fail(E2D_ERR_ARG, "max_steps must be > 0");
+  if (!b->ready) return fail(E2D_ERR_ARG, "e2d_batch_init has not run on this batch");
   if (use_graph) {
     rc = batch_graph(b);
     if (rc) return rc;
@@ -1250,6 +1254,7 @@ int e2d_batch_autotune(e2d_batch *b, const int *candidates, int ncand, int steps
   int rc = batch_check(b);
   if (rc) return rc;
   if (!candidates || ncand <= 0 || steps <= 0) return fail(E2D_ERR_ARG, "bad argument");
+  if (!b->ready) return fail(E2D_ERR_ARG, "e2d_batch_init has not run on this batch");
   int best = b->seg_rows;
   double best_ms = 1e300;
   // two interleaved passes; the faster timing of each candidate counts

@@ -250,7 +250,8 @@ int e2d_batch_init(e2d_batch *batch);
 int e2d_batch_set_tile(e2d_batch *batch, int seg_rows);
 /* Runs up to max_steps batched steps (instances whose loop has ended stay put).
  * use_graph != 0 replays captured hipGraphs of the step launch.  Returns the
- * device time in milliseconds and the number of instances still active. */
+ * device time in milliseconds and the number of instances still active.
+ * E2D_ERR_ARG until e2d_batch_init has run (as for e2d_batch_autotune). */
 int e2d_batch_run(e2d_batch *batch, int max_steps, int use_graph, double *elapsed_ms,
                   int *active_left);
 int e2d_batch_status(e2d_batch *batch, int instance, double *t, int *nStep);

@@ -104,6 +104,21 @@ def test_batch_fp64_equals_oracle(gpu_available):
         np.testing.assert_array_equal(batch.state(i), ora.current())
 
 
+def test_batch_run_requires_init(gpu_available):
+    """A batch whose instances have no initial state and no startup dt yet
+    refuses to step, then runs once e2d_batch_init has set them."""
+    from euler2d_kokkos_amd._capi import E2DError
+    ps = mixed_instances()[:2]
+    batch = HydroBatch(ps, precision=E2D_PRECISION_FP64, initialise=False)
+    with pytest.raises(E2DError, match="e2d_batch_init"):
+        batch.run(3)
+    with pytest.raises(E2DError, match="e2d_batch_init"):
+        batch.autotune(candidates=(8,), steps=1)
+    batch.init()
+    batch.run(3)
+    assert batch.status(0)[1] == 3
+
+
 def test_batch_autotune_and_reinit(gpu_available):
     ps = mixed_instances()
     batch = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
