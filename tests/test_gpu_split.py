@@ -58,6 +58,34 @@ def test_split_close_to_fp64(gpu_available, case):
     assert abs(r32.t - r64.t) <= 1e-4 * r64.t
 
 
+def test_split_two_wavefronts_per_simd(gpu_available):
+    """The blast workload's grid, 1024 x 1536, at the 14-row tile (2016
+    wavefronts, two per SIMD on a 256-CU MI355X, with the progress-based issue
+    priority): the split run stays within the L1 gate of the FP64 run, and on
+    the closed domain conserves mass and energy."""
+    p = O.default_params(nx=1024, ny=1536, ymax=1.5, problemType=1, cfl=0.7, nStepmax=100,
+                         tEnd=1e9, blast_radius=0.1, blast_center_x=0.5, blast_center_y=0.75,
+                         blast_density_in=1.0, blast_density_out=1.0, blast_pressure_in=10.0,
+                         blast_pressure_out=0.1)
+    runs = {}
+    for prec in (E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT):
+        r = HydroRun(HydroParams(p.copy()), precision=prec)
+        r.set_tile(14)
+        runs[prec] = r
+    r64, r32 = runs[E2D_PRECISION_FP64], runs[E2D_PRECISION_FP32_SPLIT]
+    s0 = r32.conserved_sums(E2D_U)
+    r64.run()
+    r32.run()
+    assert r32.nStep == r64.nStep == 100
+    err = O.rel_l1(O.interior(r32.get_state(r32.current())),
+                   O.interior(r64.get_state(r64.current())))
+    s1 = r32.conserved_sums(r32.current())
+    print(f"1024x1536/14: relL1={err.tolist()} mass/energy drift={(s1 - s0)[[0, 1]].tolist()}")
+    assert err[0] <= L1_TOL, err
+    for v in (0, 1):
+        assert abs(s1[v] - s0[v]) <= 1e-12 * abs(s0[v]), (v, s0, s1)
+
+
 @pytest.mark.parametrize("bc,conserved", [(1, (0, 1)), (3, (0, 1, 2, 3))])
 def test_split_conservation(gpu_available, bc, conserved):
     p = O.default_params(nx=160, ny=120, problemType=1, cfl=0.7, nStepmax=1000, tEnd=1e9,
