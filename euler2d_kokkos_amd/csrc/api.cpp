@@ -1071,11 +1071,11 @@ int batch_state(e2d_batch *b, int i, e2d::StepState &s) {
 }
 
 int batch_active(e2d_batch *b, int *active) {
+  // one strided copy gathers entry `phase` of every instance's rings
   std::vector<e2d::StepState> s(b->runs.size());
-  for (size_t i = 0; i < s.size(); ++i) {
-    int rc = batch_state(b, (int)i, s[i]);
-    if (rc) return rc;
-  }
+  HIP_TRY(hipMemcpy2DAsync(s.data(), sizeof(e2d::StepState), &b->ctls[0].s[b->phase],
+                           sizeof(e2d::StepCtl), sizeof(e2d::StepState), s.size(),
+                           hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   int n = 0;
   for (size_t i = 0; i < s.size(); ++i) n += loop_view(b->runs[i]->p, s[i]).active ? 1 : 0;
