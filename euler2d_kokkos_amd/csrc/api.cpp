@@ -65,6 +65,9 @@ struct e2d_run {
   Grid U{}, U2{}, Q{}, Fx{}, Fy{}, Sx{}, Sy{};
   e2d::StepCtl *ctl = nullptr;
   int phase = 0;  // phase of the next run-loop launch in the control rings
+  // captured run of kLoopGraphSteps run-loop launches (for seg_rows = loop_graph_tile)
+  hipGraphExec_t loop_graph = nullptr;
+  int loop_graph_tile = 0;
   unsigned long long *bits = nullptr;
   unsigned int *count = nullptr;
   double *partials = nullptr;
@@ -219,14 +222,58 @@ int loop_begin(e2d_run *r, double t, int nStep) {
   return E2D_OK;
 }
 
+// Run-loop launches per captured graph: a multiple of 6, so that a replay that
+// starts at ring phase 0 on an even step ends there (phase ring 3, array parity 2).
+constexpr int kLoopGraphSteps = 30;
+
+int loop_graph(e2d_run *r) {
+  if (r->loop_graph && r->loop_graph_tile == r->seg_rows) return E2D_OK;
+  if (r->loop_graph) (void)hipGraphExecDestroy(r->loop_graph);
+  r->loop_graph = nullptr;
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < kLoopGraphSteps; ++k) {
+    const Grid &in = (k % 2 == 0) ? r->U : r->U2;
+    const Grid &out = (k % 2 == 0) ? r->U2 : r->U;
+    hipError_t e = e2d::launch_step_fused(r->kp, in, out, 0.0, r->ctl, k % 3, r->seg_rows,
+                                          r->stream);
+    if (e != hipSuccess) {
+      (void)hipStreamEndCapture(r->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(e, "launch_step_fused (capture)");
+    }
+  }
+  HIP_TRY(hipStreamEndCapture(r->stream, &g));
+  hipError_t e = hipGraphInstantiate(&r->loop_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    r->loop_graph = nullptr;
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  r->loop_graph_tile = r->seg_rows;
+  return E2D_OK;
+}
+
+// Queues `count` run-loop launches from step nStep0 on: captured runs of
+// kLoopGraphSteps where the ring phase is 0 and the step even, single launches
+// elsewhere.
 int loop_steps(e2d_run *r, int nStep0, int count) {
-  for (int k = 0; k < count; ++k) {
+  int k = 0;
+  while (k < count) {
     const int n = nStep0 + k;
+    if (count - k >= kLoopGraphSteps && r->phase == 0 && n % 2 == 0) {
+      int rc = loop_graph(r);
+      if (rc) return rc;
+      HIP_TRY(hipGraphLaunch(r->loop_graph, r->stream));
+      k += kLoopGraphSteps;
+      continue;
+    }
     Grid in = (n % 2 == 0) ? r->U : r->U2;
     Grid out = (n % 2 == 0) ? r->U2 : r->U;
     HIP_TRY(e2d::launch_step_fused(r->kp, in, out, 0.0, r->ctl, r->phase, r->seg_rows,
                                    r->stream));
     r->phase = r->phase == 2 ? 0 : r->phase + 1;
+    ++k;
   }
   return E2D_OK;
 }
@@ -639,6 +686,7 @@ int e2d_destroy(e2d_run *run) {
   if (!run) return fail(E2D_ERR_ARG, "null e2d_run handle");
   (void)hipSetDevice(run->device);
   if (run->stream) (void)hipStreamSynchronize(run->stream);
+  if (run->loop_graph) (void)hipGraphExecDestroy(run->loop_graph);
   for (void *ptr : run->allocs) (void)hipFree(ptr);
   if (run->stream) (void)hipStreamDestroy(run->stream);
   delete run;
