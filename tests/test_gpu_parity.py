@@ -238,6 +238,24 @@ def test_loop_in_pieces_matches_oracle(gpu_available):
     assert run.t == p.tEnd
 
 
+def test_loop_graph_replays_match_oracle(gpu_available):
+    """Run-loop calls of 30+ steps replay a captured graph of 30 launches where
+    the control ring's phase is 0 and the step even.  Every replay is checked
+    against the oracle at the same step, with eager single launches, state
+    downloads and a tile change (a new capture) in between."""
+    p = _params("implode64")
+    p.nStepmax = 400
+    ora = O.OracleRun(p)
+    run = HydroRun(HydroParams(p.copy()))
+    run.set_tile(9)
+    for k, tile in ((30, None), (3, None), (33, None), (60, 16), (1, None), (61, None)):
+        if tile:
+            run.set_tile(tile)
+        run.run(k)
+        ora.run(k)
+        _compare(run, ora, f"graph/{run.nStep}")
+
+
 def test_autotune_restores_state(gpu_available):
     p = _params("blast48x72")
     run = HydroRun(HydroParams(p.copy()))
