@@ -1,6 +1,6 @@
 """FP64 HIP path against the CPU oracle on the same seeded inputs.
 
-Gate: relative L1 <= 1e-12 per conserved variable after N
+Gate of the FP64 drop-in: relative L1 <= 1e-12 per conserved variable after N
 steps.  The FP64 kernels are built to reproduce the oracle bit for bit, so the
 tests also check exact equality of the full arrays (ghost cells included) and
 of the accumulated time t.

@@ -1,6 +1,6 @@
 """FP32 split-precision path against the FP64 path on the GPU.
 
-Gates (north star): relative L1 density error <= 1e-4 against FP64 after N
+Gates of the split path: relative L1 density error <= 1e-4 against FP64 after N
 steps, and total mass and energy conserved to ~1e-12 (FP64 sums of hi + lo).
 """
 import numpy as np
