@@ -880,11 +880,9 @@ __device__ __forceinline__ void set_prio_level(int lvl) {
 template <int RS, class P, bool RATE, bool O32, bool COLS>
 __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in, const Grid &out,
                                                double dt, int seg_rows, int nstrips, int strip,
-                                               int seg, bool prio) {
+                                               int seg, bool prio, bool go = true) {
   using T = typename P::T;
   const int lane = threadIdx.x & (kLanes - 1);
-  const T dtdx = (T)(dt / kp.dx);
-  const T dtdy = (T)(dt / kp.dy);
 
   const int nsegs = (kp.ny + seg_rows - 1) / seg_rows;
   const int i_base = 2 + kOutCols * strip;
@@ -925,6 +923,11 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
     }
     if (flipy) P::negate(m.un, IV);
   }
+  // go = false (run loop ended): nothing is stored.  The check comes after the
+  // first row's load is issued, so that its latency overlaps the loop-state read.
+  if (!go) return T(0.0);
+  const T dtdx = (T)(dt / kp.dx);
+  const T dtdy = (T)(dt / kp.dy);
   m.ua = m.un;
   m.ub = m.un;
   m.qm2 = m.qm1 = m.fymax_c = m.fx_c = m.fy_c = Q4T<T>{};
@@ -985,12 +988,17 @@ __device__ __forceinline__ LoopPos loop_pos(const KParams &kp, const StepCtl *ct
   L.t = ctl->s[ph].t;
   L.nstep = ctl->s[ph].nstep;
   L.active = L.t < kp.tEnd && L.nstep < kp.nStepmax;
-  unsigned long long m = 0ull;  // max of positive doubles = max of their bits
-  for (int k = 0; k < kRateSlots; ++k) {
-    const unsigned long long b = ctl->rate[ph][k * kSlotStride];
-    m = b > m ? b : m;
-  }
-  double dt = kp.cfl / __longlong_as_double((long long)m);
+  // one vector load reads the 16 slots (16 lines) at once: lane k reads slot k.
+  // The slots hold +0 or positive rates, so the max of the doubles is the max
+  // of their bits.
+  const int lane = threadIdx.x & (kLanes - 1);
+  const double b =
+      lane < kRateSlots ? __longlong_as_double((long long)ctl->rate[ph][lane * kSlotStride]) : 0.0;
+  const long long mb = __double_as_longlong(wave_max(b));
+  const unsigned mlo = __builtin_amdgcn_readfirstlane((unsigned)mb);
+  const unsigned mhi = __builtin_amdgcn_readfirstlane((unsigned)(mb >> 32));
+  const double invdt = __longlong_as_double((long long)(((unsigned long long)mhi << 32) | mlo));
+  double dt = kp.cfl / invdt;
   if (L.t + dt > kp.tEnd) dt = kp.tEnd - L.t;
   L.dt = dt;
   return L;
@@ -1083,10 +1091,11 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
                                                             FusedMap fm) {
   using T = typename P::T;
   double dt = dt_arg;
+  bool go = true;
   if constexpr (LOOP) {
     const LoopPos L = loop_pos(kp, ctl, ph);
     if (blockIdx.x == 0 && threadIdx.x < kLanes) loop_publish(ctl, ph, L, threadIdx.x);
-    if (!L.active) return;  // loop finished: nothing to do (uniform)
+    go = L.active;  // false once the loop has finished (uniform over the grid)
     dt = L.dt;
   }
   // the wave index is uniform: every row index and the row ghost map derived
@@ -1103,13 +1112,16 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
     o.p = out.p;
     if (interior_cols(kp, strip))  // uniform
       wmax = march<RS, P, LOOP, O32, false>(kp, in, o, dt, tile, fm.nstrips, strip, seg,
-                                            fm.prio != 0);
+                                            fm.prio != 0, go);
     else
       wmax = march<RS, P, LOOP, O32, true>(kp, in, o, dt, tile, fm.nstrips, strip, seg,
-                                           fm.prio != 0);
+                                           fm.prio != 0, go);
   }
   // run loop: the CFL rate of the step's output, for the next launch's dt
-  if constexpr (LOOP) reduce_rate_block(wmax, rate_slot(ctl, ph));
+  if constexpr (LOOP) {
+    if (!go) return;
+    reduce_rate_block(wmax, rate_slot(ctl, ph));
+  }
 }
 
 // ------------------------------------------------------- batched step kernel
