@@ -207,7 +207,11 @@ int e2d_autotune(e2d_run *run, const int *candidates, int ncand, int steps, int 
 int e2d_synchronize(e2d_run *run);
 /* Device-side timing of `nsteps` loop iterations (e2d_run_loop semantics) with
  * HIP events on the handle's stream; returns the elapsed milliseconds and the
- * number of steps actually executed.  Used by benchmarks. */
+ * number of steps actually executed.  Used by benchmarks.  Runs of 30 or more
+ * steps replay a captured graph of 30 launches; the first such run on a handle
+ * (and the first after a tile change) captures and instantiates it, which
+ * takes a few milliseconds: warm up with e2d_run_loop over 30 or more steps
+ * before timing. */
 int e2d_time_loop(e2d_run *run, int nsteps, double *elapsed_ms, int *executed, double *t,
                   int *nStep);
 
