@@ -1,11 +1,13 @@
 """Batched driver on the GPU: every instance of a batch must equal the same
 problem run alone (bit for bit: the batched kernel runs the same march per
-wavefront), in FP64 (and hence the CPU oracle) and in split precision."""
+wavefront), in FP64 (and hence the CPU oracle), in split precision and in the
+plain FP32 comparator."""
 import numpy as np
 import pytest
 
 import oracle_lib as O
-from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64, HydroParams, HydroRun
+from euler2d_kokkos_amd import (E2D_PRECISION_FP32, E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64,
+                                HydroParams, HydroRun)
 from euler2d_kokkos_amd.batch import HydroBatch, make_params
 
 pytestmark = pytest.mark.gpu
@@ -28,8 +30,8 @@ def mixed_instances():
     ]
 
 
-@pytest.mark.parametrize("precision", [E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT],
-                         ids=["fp64", "split"])
+@pytest.mark.parametrize("precision", [E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT,
+                                       E2D_PRECISION_FP32], ids=["fp64", "split", "fp32"])
 @pytest.mark.parametrize("use_graph", [False, True], ids=["launches", "graph"])
 def test_batch_equals_single_runs(gpu_available, precision, use_graph):
     ps = mixed_instances()
