@@ -1021,41 +1021,22 @@ __device__ __forceinline__ void loop_publish(StepCtl *ctl, int ph, const LoopPos
   }
 }
 
-// The rate slot block b of launch phase ph folds into.
+// The rate slot the calling wavefront of launch phase ph folds into: slots are
+// spread by global wavefront index.
 __device__ __forceinline__ unsigned long long *rate_slot(StepCtl *ctl, int ph) {
   const int p1 = ph == 2 ? 0 : ph + 1;
-  return &ctl->rate[p1][(blockIdx.x % kRateSlots) * kSlotStride];
+  const unsigned wave = blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6);
+  return &ctl->rate[p1][(wave % kRateSlots) * kSlotStride];
 }
 
-// Folds the wave maxima of a block into their targets: one device-scope
-// atomic per block and distinct target (the max is exact and order
-// independent).  In the batched kernel the waves of one block can belong to
-// different instances, hence the merge per target.
+// Folds the wavefront's maximum CFL rate into *target: one device-scope
+// atomic per wavefront (the max is exact and order independent).  No barrier
+// and no LDS merge: a wavefront that has finished its march leaves at once.
+// A NaN rate never reaches a slot (m > 0 is false for it).
 template <class T>
-__device__ __forceinline__ void reduce_rate_block(T wmax, unsigned long long *target) {
-  constexpr int kWaves = kBlock / kLanes;
-  __shared__ double wm[kWaves];
-  __shared__ unsigned long long *wp[kWaves];
+__device__ __forceinline__ void reduce_rate_wave(T wmax, unsigned long long *target) {
   const double m = (double)wave_max(wmax);
-  if ((threadIdx.x & (kLanes - 1)) == 0) {
-    const int w = threadIdx.x / kLanes;
-    wm[w] = m;
-    wp[w] = target;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 0; w < kWaves; w++) {
-      unsigned long long *p = wp[w];
-      if (!p) continue;
-      double b = 0.0;
-      for (int v = w; v < kWaves; v++)
-        if (wp[v] == p) {
-          if (wm[v] == wm[v] && wm[v] > b) b = wm[v];
-          wp[v] = nullptr;
-        }
-      if (b > 0.0) atomicMax(p, dbits(b));
-    }
-  }
+  if ((threadIdx.x & (kLanes - 1)) == 0 && target && m > 0.0) atomicMax(target, dbits(m));
 }
 
 // True when every column the wavefront of strip `strip` loads (60 strip ..
@@ -1120,7 +1101,7 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
   // run loop: the CFL rate of the step's output, for the next launch's dt
   if constexpr (LOOP) {
     if (!go) return;
-    reduce_rate_block(wmax, rate_slot(ctl, ph));
+    reduce_rate_wave(wmax, rate_slot(ctl, ph));
   }
 }
 
@@ -1174,7 +1155,7 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__r
       target = rate_slot(ctl, ph);
     }
   }
-  reduce_rate_block(wmax, target);
+  reduce_rate_wave(wmax, target);
 }
 
 // ----------------------------------------------------------- radial profile

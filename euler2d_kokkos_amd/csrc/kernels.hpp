@@ -50,15 +50,16 @@ struct KParams {
 // (phase ph = L % 3) reads s[ph], the time and step count before its step, and
 // rate[ph], the CFL rate of its input, from which every wavefront derives the
 // same dt (HydroRun.h:246, with the tEnd clamp) and while condition.  One
-// wavefront writes s[(ph + 1) % 3] and clears rate[(ph + 2) % 3]; the blocks
-// fold the rate of the step's output into rate[(ph + 1) % 3].  No launch reads
-// what it writes, so no block waits for another at the end of a step.
+// wavefront writes s[(ph + 1) % 3] and clears rate[(ph + 2) % 3]; every
+// wavefront folds the rate of its part of the step's output into
+// rate[(ph + 1) % 3].  No launch reads what it writes, so nothing waits for
+// another wavefront at the end of a step.
 struct StepState {
   double t;   // time before the step
   int nstep;  // steps executed before it
   int pad;
 };
-constexpr int kRateSlots = 16;   // block b folds into slot b % kRateSlots
+constexpr int kRateSlots = 16;   // wavefront w folds into slot w % kRateSlots
 constexpr int kSlotStride = 16;  // one 128-byte line per slot
 struct StepCtl {
   StepState s[3];
