@@ -16,8 +16,13 @@ the algorithmic 64 bytes per cell-update (read 4 doubles, write 4 doubles).
 `fp32_split` repeats the run in split precision (hi+lo FP32 state) and reports
 its distance from the FP64 result; `batched` times the seeded 512-instance
 ragged sweep (sizes 64..512 per side, FP32 split, 1000 steps each) advanced by
-one batched launch per step.  `cpu_baseline` times the CPU oracle (a restatement of the reference's
-implementation 0 with OpenMP) on this host over a bounded sample.
+one batched launch per step, and reports every instance's L1 density error
+against the same sweep run in FP64; `configs` times the other decks (implode
+256x256, four-quadrant 8192x8192, shocked bubble 16384x8192) over a bounded
+number of steps in both precisions, with the split run's L1 error and drift.
+`cpu_baseline` times the CPU oracle (a restatement of the reference's
+implementation 0 with OpenMP) on this host: the median of three bounded
+samples, with the host's CPU model, socket count and thread placement.
 """
 from __future__ import annotations
 
@@ -27,6 +32,11 @@ import os
 import sys
 import time
 from pathlib import Path
+
+# OpenMP thread placement of the CPU baseline; read when the OpenMP runtime
+# starts, so set before any library that links it is loaded
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -63,6 +73,27 @@ def profiled_traffic(workload: str, tile: int):
     return None
 
 
+# VALU issue peak: one wave64 VALU instruction per SIMD every 4 cycles (16
+# lanes per cycle), 1024 SIMDs at the 2.4 GHz peak engine clock; for FP64 FMAs
+# this is the 78.6 TFLOP/s vector peak
+VALU_PEAK_INSTS = 1024 * 2.4e9 / 4
+
+
+def profiled_valu(workload: str, tile: int):
+    """SQ_INSTS_VALU per launch of the FP64 step kernel from the committed
+    rocprofv3 counter summary of this workload and tile, if there is one."""
+    import re
+    if workload != "blast":
+        return None
+    for path in sorted(PROFILES.glob("step_fused_fp64_blast_1024x1536*.md")):
+        text = path.read_text()
+        m = re.search(r"^valu_insts_per_launch: (\d+)", text, re.M)
+        t = re.search(r"seg_rows (\d+)", text)
+        if m and t and int(t.group(1)) == tile:
+            return int(m.group(1))
+    return None
+
+
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +112,10 @@ def parse_args():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 512-instance sweep")
     ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other decks (implode, four-quadrant, shocked bubble)")
+    ap.add_argument("--config-steps", type=int, default=100,
+                    help="steps of each other deck in the `configs` record")
     ap.add_argument("--workload", default="blast",
                     choices=["blast", "four_quadrant", "shocked_bubble", "implode"],
                     help="deck to time (blast 1024x1536 is the headline workload)")
@@ -92,66 +127,140 @@ def parse_args():
 
 def batch_line(steps: int) -> dict:
     """The 512-instance ragged sweep (FP32 split, HLLC, 1000 steps each) in one
-    batched launch per step, replayed from hipGraphs; tile autotuned first."""
-    from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT
-    from euler2d_kokkos_amd.batch import HydroBatch, sweep_params
+    batched launch per step, replayed from hipGraphs; tile autotuned first.
+    The same sweep then runs in FP64 (also timed), and every instance's
+    relative L1 density error against its FP64 twin is reported."""
+    import numpy as np
+
+    from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64
+    from euler2d_kokkos_amd.batch import HydroBatch, density_l1, sweep_params
 
     ps = sweep_params(steps=steps)
-    b = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
-    tile, _ = b.autotune(candidates=(8, 12, 16, 24, 32, 48, 64), steps=5)
-    b.init()
-    info = b.info()
-    t0 = time.perf_counter()
-    ms, active = b.run(steps, use_graph=True)
-    wall = time.perf_counter() - t0
-    done = sum(b.status(i)[1] for i in range(info["count"]))
-    cell_updates = sum(p.nx * p.ny * b.status(i)[1] for i, p in enumerate(ps))
-    b.close()
-    return {"value": round(cell_updates / wall / 1e6, 2), "unit": "Mcell-updates/s",
+    cand = (8, 12, 16, 24, 32, 48, 64)
+    out = {}
+    runs = {}
+    for name, prec in (("fp32_split", E2D_PRECISION_FP32_SPLIT), ("fp64", E2D_PRECISION_FP64)):
+        b = HydroBatch(ps, precision=prec)
+        tile, _ = b.autotune(candidates=cand, steps=5)
+        b.init()
+        info = b.info()
+        t0 = time.perf_counter()
+        ms, active = b.run(steps, use_graph=True)
+        wall = time.perf_counter() - t0
+        done = sum(b.status(i)[1] for i in range(info["count"]))
+        cell_updates = sum(p.nx * p.ny * b.status(i)[1] for i, p in enumerate(ps))
+        out[name] = {"value": round(cell_updates / wall / 1e6, 2), "tile": tile,
+                     "device_ms": round(ms, 3), "instance_steps_done": done,
+                     "still_active": active}
+        runs[name] = b
+    err = density_l1(runs["fp32_split"], runs["fp64"])
+    worst = int(np.argmax(err))
+    pw = ps[worst]
+    for b in runs.values():
+        b.close()
+    split = out["fp32_split"]
+    return {"value": split["value"], "unit": "Mcell-updates/s",
             "instances": info["count"], "cells": info["cells"], "steps_each": steps,
-            "instance_steps_done": done, "still_active": active, "tile": tile,
-            "device_ms": round(ms, 3), "precision": "fp32_split", "launch": "hipGraph"}
+            "instance_steps_done": split["instance_steps_done"],
+            "still_active": split["still_active"], "tile": split["tile"],
+            "device_ms": split["device_ms"], "precision": "fp32_split", "launch": "hipGraph",
+            "fp64": out["fp64"],
+            "speedup_vs_fp64": round(split["value"] / out["fp64"]["value"], 3),
+            "rel_l1_density_vs_fp64": {
+                "max": float(err.max()), "median": float(np.median(err)),
+                "mean": float(err.mean()), "instances_above_1e-4": int((err > 1e-4).sum()),
+                "worst": {"instance": worst, "nx": pw.nx, "ny": pw.ny,
+                          "problem": "implode" if pw.problemType == 0 else "blast",
+                          "pressure_ratio": round(pw.blast_pressure_in / pw.blast_pressure_out
+                                                  if pw.problemType == 1 else
+                                                  pw.implode_pressure_in / 0.14, 2),
+                          "rel_l1": float(err[worst])}}}
+
+
+def host_cpu() -> dict:
+    """CPU model, sockets and logical CPUs of this host (/proc/cpuinfo)."""
+    model, sockets = "unknown", set()
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"model": model, "sockets": max(len(sockets), 1), "nproc": os.cpu_count() or 1,
+            "usable_cpus": usable}
 
 
 def cpu_baseline(deck: Path, seconds: float) -> dict:
-    """Times the CPU oracle on this host over a bounded sample of the workload."""
+    """Times the CPU oracle on this host: the median of three samples of the
+    workload, each bounded to a third of `seconds`.  Threads: OMP_NUM_THREADS
+    when set (the CPU share a GPU job gets on a shared host), else every CPU
+    this process may run on; placement OMP_PROC_BIND / OMP_PLACES."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O  # test infrastructure, used here only as the baseline
 
-    threads = int(os.environ.get("E2D_CPU_THREADS", "0")) or (os.cpu_count() or 1)
-    threads = min(threads, 16)
+    host = host_cpu()
+    threads = int(os.environ.get("E2D_CPU_THREADS") or os.environ.get("OMP_NUM_THREADS")
+                  or host["usable_cpus"])
     O.lib().ora_set_threads(threads)
     p = O.params_from_ini(deck)
     run = O.OracleRun(p)
     run.run(1)  # warm-up step (first touch of the arrays)
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds and run.nStep < p.nStepmax:
-        run.run(1)
-        steps += 1
-    el = time.perf_counter() - t0
-    value = p.nx * p.ny * steps / el / 1e6
+    rates, counts, spans = [], [], []
+    for _ in range(3):
+        steps = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds / 3 and run.nStep < p.nStepmax:
+            run.run(1)
+            steps += 1
+        el = time.perf_counter() - t0
+        rates.append(p.nx * p.ny * steps / el / 1e6)
+        counts.append(steps)
+        spans.append(round(el, 2))
+    value = sorted(rates)[1]
     return {"value": round(value, 3), "unit": "Mcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{deck.name}: {steps} steps after 1 warm-up step, CPU oracle "
-                      f"(FP64) with OpenMP over {threads} threads, {el:.1f} s"}
+            "kind": "port", "samples": [round(r, 3) for r in rates],
+            "host": host, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES"),
+            "sample": f"{deck.name}: median of 3 samples of {counts} steps ({spans} s) after 1 "
+                      f"warm-up step, CPU oracle (FP64) with OpenMP over {threads} threads"}
 
 
-def split_comparison(params, run64, steps, cells, fp64_ms, tile=0) -> dict:
-    """Same workload in FP32 split precision: throughput, L1 error against the
-    FP64 run's final state, FP64-summed drift of mass and energy over the run
-    (nonzero where the boundaries let mass through), and the distance of the
-    final mass and energy from the FP64 run's."""
+def rel_l1_runs(run, ref) -> list:
+    """Per-variable relative L1 distance of two runs' current interiors, one
+    variable at a time (sum|a - b| / sum|b|; the absolute sum where the
+    reference variable is zero everywhere, as a y velocity at t = 0)."""
     import numpy as np
 
-    from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_U, HydroRun
+    from euler2d_kokkos_amd import E2D_LAYOUT_INTERIOR
 
-    run = HydroRun(params, precision=E2D_PRECISION_FP32_SPLIT)
+    a = run.get_state(run.current(), E2D_LAYOUT_INTERIOR)
+    b = ref.get_state(ref.current(), E2D_LAYOUT_INTERIOR)
+    out = []
+    for v in range(4):
+        num = float(np.abs(a[v] - b[v]).sum())
+        den = float(np.abs(b[v]).sum())
+        out.append(num / den if den > 0 else num)
+    return out
+
+
+def timed_run(params, precision, steps, tile=0):
+    """A fresh run of the deck: tile autotuned on the initial state (or fixed),
+    30 warm-up steps (the run loop's graph is captured there), then `steps`
+    steps from t = 0 timed with the host clock and with HIP events."""
+    from euler2d_kokkos_amd import E2D_U, HydroRun
+
+    run = HydroRun(params, precision=precision)
     if tile > 0:
         run.set_tile(tile)
     else:
-        run.autotune(steps=20)
-    run.run(20)
+        run.autotune(steps=10)
+    run.run(30)
     run.synchronize()
     run.init_problem()
     s0 = run.conserved_sums(E2D_U)
@@ -159,28 +268,74 @@ def split_comparison(params, run64, steps, cells, fp64_ms, tile=0) -> dict:
     ms, executed = run.time_loop(steps)
     run.synchronize()
     wall = time.perf_counter() - t0
-    a = run.get_state(run.current())[2:-2, 2:-2, :].reshape(-1, 4)
-    b = run64.get_state(run64.current())[2:-2, 2:-2, :].reshape(-1, 4)
-    num = np.abs(a - b).sum(0)
-    den = np.abs(b).sum(0)
-    # a variable that is zero everywhere in the FP64 result (no y velocity yet)
-    # reports its absolute L1 difference
-    l1 = np.where(den > 0, num / np.where(den > 0, den, 1.0), num)
+    return run, s0, ms, executed, wall
+
+
+def split_comparison(params, run64, steps, cells, fp64_ms, tile=0) -> dict:
+    """Same workload in FP32 split precision: throughput, L1 error against the
+    FP64 run's final state, FP64-summed drift of mass and energy over the run
+    (nonzero where the boundaries let mass through), and the distance of the
+    final mass and energy from the FP64 run's."""
+    from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT
+
+    run, s0, ms, executed, wall = timed_run(params, E2D_PRECISION_FP32_SPLIT, steps, tile)
+    l1 = rel_l1_runs(run, run64)
     s1 = run.conserved_sums(run.current())
     s64 = run64.conserved_sums(run64.current())
     value = cells * executed / wall / 1e6
     per_step = ms / executed
-    return {"value": round(value, 2), "unit": "Mcell-updates/s",
-            "speedup_vs_fp64": round(fp64_ms / per_step, 3),
-            "kernel_ms_per_step": round(per_step, 5),
-            "roofline_frac": round(BYTES_PER_CELL * cells / (per_step * 1e-3) / 1e9
-                                   / HBM_PEAK_GBS, 4),
-            "rel_l1_vs_fp64": [float(x) for x in l1],
-            "mass_drift": float(abs(s1[0] - s0[0]) / abs(s0[0])),
-            "energy_drift": float(abs(s1[1] - s0[1]) / abs(s0[1])),
-            "mass_vs_fp64": float(abs(s1[0] - s64[0]) / abs(s64[0])),
-            "energy_vs_fp64": float(abs(s1[1] - s64[1]) / abs(s64[1])),
-            "tile": run.get_tile(), "final_t": run.t}
+    out = {"value": round(value, 2), "unit": "Mcell-updates/s",
+           "speedup_vs_fp64": round(fp64_ms / per_step, 3),
+           "kernel_ms_per_step": round(per_step, 5),
+           "roofline_frac": round(BYTES_PER_CELL * cells / (per_step * 1e-3) / 1e9
+                                  / HBM_PEAK_GBS, 4),
+           "rel_l1_vs_fp64": l1,
+           "mass_drift": float(abs(s1[0] - s0[0]) / abs(s0[0])),
+           "energy_drift": float(abs(s1[1] - s0[1]) / abs(s0[1])),
+           "mass_vs_fp64": float(abs(s1[0] - s64[0]) / abs(s64[0])),
+           "energy_vs_fp64": float(abs(s1[1] - s64[1]) / abs(s64[1])),
+           "kinetic_cells": run.kinetic_cells(run.current()),
+           "tile": run.get_tile(), "final_t": run.t}
+    run.close()
+    return out
+
+
+def config_pair(workload: str, steps: int) -> dict:
+    """One of the other decks in FP64 and in split precision over `steps`
+    steps from t = 0 (the deck's own count is larger; the grid is full size):
+    throughput of each, the split run's L1 error against the FP64 run, and its
+    mass / energy drift."""
+    from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64, HydroParams
+
+    deck = ROOT / "euler2d_kokkos_amd" / "decks" / DECKS[workload][0]
+    params = HydroParams.from_ini(deck)
+    steps = min(steps, params.nStepmax)
+    cells = params.nx * params.ny
+    r64, _, ms64, ex64, wall64 = timed_run(params, E2D_PRECISION_FP64, steps)
+    r32, s0, ms32, ex32, wall32 = timed_run(params, E2D_PRECISION_FP32_SPLIT, steps)
+    if ex64 != steps or ex32 != steps:
+        raise RuntimeError(f"{workload}: ran {ex64} / {ex32} steps instead of {steps}")
+    s1 = r32.conserved_sums(r32.current())
+    l1 = rel_l1_runs(r32, r64)
+    k64, k32 = ms64 / steps, ms32 / steps
+    out = {"deck": DECKS[workload][0], "nx": params.nx, "ny": params.ny, "steps": steps,
+           "deck_steps": DECKS[workload][2],
+           "fp64": {"value": round(cells * steps / wall64 / 1e6, 2),
+                    "kernel_ms_per_step": round(k64, 5), "tile": r64.get_tile(),
+                    "roofline_frac": round(BYTES_PER_CELL * cells / (k64 * 1e-3) / 1e9
+                                           / HBM_PEAK_GBS, 4)},
+           "fp32_split": {"value": round(cells * steps / wall32 / 1e6, 2),
+                          "kernel_ms_per_step": round(k32, 5), "tile": r32.get_tile(),
+                          "roofline_frac": round(BYTES_PER_CELL * cells / (k32 * 1e-3) / 1e9
+                                                 / HBM_PEAK_GBS, 4)},
+           "speedup_split_vs_fp64": round(k64 / k32, 3),
+           "rel_l1_split_vs_fp64": l1,
+           "split_mass_drift": float(abs(s1[0] - s0[0]) / abs(s0[0])),
+           "split_energy_drift": float(abs(s1[1] - s0[1]) / abs(s0[1])),
+           "split_kinetic_cells": r32.kinetic_cells(r32.current())}
+    r32.close()
+    r64.close()
+    return out
 
 
 def main() -> int:
@@ -235,6 +390,12 @@ def main() -> int:
     value = cells * args.steps / wall / 1e6
     per_step_ms = kernel_ms / args.steps
     achieved = BYTES_PER_CELL * cells / (per_step_ms * 1e-3) / 1e9
+    # the second roof: VALU instructions per launch (committed counter summary
+    # of this kernel, workload and tile) issued in this run's kernel time,
+    # against one wave64 instruction per SIMD every 4 cycles
+    valu_insts = profiled_valu(args.workload, run.get_tile())
+    valu_frac = (round(valu_insts / (per_step_ms * 1e-3) / VALU_PEAK_INSTS, 4)
+                 if valu_insts else None)
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -253,7 +414,8 @@ def main() -> int:
                    "tile": run.get_tile(), "device": dev, "cus": ncu},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(args.workload, run.get_tile())},
+                     "traffic": profiled_traffic(args.workload, run.get_tile()),
+                     "valu_frac": valu_frac, "valu_insts_per_launch": valu_insts},
         "kernel_ms_per_step": round(per_step_ms, 5),
         "final_t": run.t,
     }
@@ -262,6 +424,10 @@ def main() -> int:
                                               args.split_tile)
     if not args.no_batch:
         line["batched"] = batch_line(args.batch_steps)
+    if not args.no_configs:
+        line["configs"] = {w: config_pair(w, args.config_steps)
+                           for w in ("implode", "four_quadrant", "shocked_bubble")
+                           if w != args.workload}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(deck, args.cpu_seconds)
     else:

@@ -60,6 +60,7 @@ PARAM_FIELDS = [
     ("sb_postshock_density", _D), ("sb_postshock_pressure", _D), ("sb_postshock_velocity", _D),
     ("sb_shock_loc", _D),
     ("implode_pressure_in", _D),
+    ("kin_frac", _D),
 ]
 
 
@@ -113,6 +114,7 @@ _PROTOTYPES = {
     "e2d_copy_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "e2d_conserved_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "e2d_timers": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "e2d_kinetic_cells": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]),
     "e2d_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "e2d_save_vtk": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
     "e2d_write_vti": (C.c_int, [C.POINTER(E2DParams), C.c_void_p, C.c_int, C.c_char_p,

@@ -124,3 +124,18 @@ class HydroBatch:
         check(load_library().e2d_batch_autotune(self._h, cand, len(candidates), steps,
                                                 C.byref(best), C.byref(ms)))
         return best.value, ms.value
+
+
+def density_l1(a: HydroBatch, b: HydroBatch) -> np.ndarray:
+    """Relative L1 density error over the interior of every instance of batch
+    `a` against the same instance of batch `b` (the same parameter list, e.g.
+    a split-precision batch against its FP64 twin): sum|rho_a - rho_b| /
+    sum|rho_b| per instance."""
+    if len(a.params) != len(b.params):
+        raise ValueError("the two batches hold different instance counts")
+    out = np.empty(len(a.params))
+    for i in range(len(a.params)):
+        ra = a.state(i, _capi.E2D_LAYOUT_INTERIOR)[0]
+        rb = b.state(i, _capi.E2D_LAYOUT_INTERIOR)[0]
+        out[i] = np.abs(ra - rb).sum() / np.abs(rb).sum()
+    return out

@@ -815,6 +815,20 @@ int e2d_conserved_sums(e2d_run *r, int which, double sums[4]) {
   return E2D_OK;
 }
 
+int e2d_kinetic_cells(e2d_run *r, int which, long long *count) {
+  int rc = check_run(r);
+  if (rc) return rc;
+  if (!count || (which != E2D_U && which != E2D_U2)) return fail(E2D_ERR_ARG, "bad argument");
+  unsigned long long *dev = r->bits + 1;  // bits[0] is compute_dt's
+  HIP_TRY(hipMemsetAsync(dev, 0, sizeof(unsigned long long), r->stream));
+  HIP_TRY(e2d::launch_kinetic_count(r->kp, state(r, which), dev, r->stream));
+  unsigned long long n = 0;
+  HIP_TRY(hipMemcpyAsync(&n, dev, sizeof(n), hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  *count = (long long)n;
+  return E2D_OK;
+}
+
 int e2d_save_vtk(e2d_run *r, int which, const char *output_dir, const char *output_prefix,
                  int step) {
   int rc = check_run(r);

@@ -126,6 +126,12 @@ struct P64 {
   static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
     prim_no_c(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], q);
   }
+  // internal energy below kin_frac of the total energy (diagnostic only)
+  static __device__ __forceinline__ bool kinetic_cell(const KParams &kp, const Cell &c) {
+    Q4 q;
+    prim(kp, c, q);
+    return !(q.v[IP] >= (double)kp.kin_frac * (kp.s.gamma0 - 1.0) * c.u[1]);
+  }
   // UpdateFunctor order: (((U + Fx_i) - Fx_{i+1}) + Fy_j) - Fy_{j+1}
   static __device__ __forceinline__ Cell update(const Cell &c, const Q4T<T> &fxl,
                                                 const Q4T<T> &fxr, const Q4T<T> &fyb,
@@ -212,15 +218,30 @@ struct PSplit {
   // FP32 conversion; cells whose internal energy is a small fraction of the
   // total energy (kinetic-dominated: p = (gamma-1)(E - rho v^2/2) cancels)
   // are converted again in FP64 from hi + lo.
-  static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
+  static __device__ __forceinline__ void prim32(const KParams &kp, const Cell &c, Q4T<T> &q) {
     prim_no_c(kp.sf, c.hi[0] + c.lo[0], c.hi[1] + c.lo[1], c.hi[2] + c.lo[2], c.hi[3] + c.lo[3],
               q);
+  }
+  // the FP32 pressure below kin_frac (gamma-1) E: internal energy below that
+  // fraction of the total (false for every cell when kin_frac = 0)
+  static __device__ __forceinline__ bool kinetic(const KParams &kp, const Cell &c,
+                                                 const Q4T<T> &q) {
     const float etot = c.hi[1];
-    if (!(q.v[IP] >= kp.kin_frac * (kp.sf.gamma0 - 1.0f) * etot)) {
+    return !(q.v[IP] >= kp.kin_frac * (kp.sf.gamma0 - 1.0f) * etot);
+  }
+  static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
+    prim32(kp, c, q);
+    if (kinetic(kp, c, q)) {
       Q4 q64;
       prim_no_c(kp.s, value(c, 0), value(c, 1), value(c, 2), value(c, 3), q64);
       for (int v = 0; v < 4; v++) q.v[v] = (float)q64.v[v];
     }
+  }
+  // the cell takes the FP64 conversion
+  static __device__ __forceinline__ bool kinetic_cell(const KParams &kp, const Cell &c) {
+    Q4T<T> q;
+    prim32(kp, c, q);
+    return kinetic(kp, c, q);
   }
   static __device__ __forceinline__ Cell update(const Cell &c, const Q4T<T> &fxl,
                                                 const Q4T<T> &fxr, const Q4T<T> &fyb,
@@ -281,6 +302,11 @@ struct P32 {
   static __device__ __forceinline__ const RateT<T> &rate(const KParams &kp) { return kp.rf; }
   static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
     prim_no_c(kp.sf, c.u[0], c.u[1], c.u[2], c.u[3], q);
+  }
+  static __device__ __forceinline__ bool kinetic_cell(const KParams &kp, const Cell &c) {
+    Q4T<T> q;
+    prim(kp, c, q);
+    return !(q.v[IP] >= kp.kin_frac * (kp.sf.gamma0 - 1.0f) * c.u[1]);
   }
   static __device__ __forceinline__ Cell update(const Cell &c, const Q4T<T> &fxl,
                                                 const Q4T<T> &fxr, const Q4T<T> &fyb,
@@ -506,6 +532,20 @@ __global__ __launch_bounds__(256) void dt_reduce_kernel(KParams kp, Grid U,
       if (wm[w] == wm[w] && wm[w] > b) b = wm[w];
     if (b > 0.0) atomicMax(invdt_bits, dbits(b));
   }
+}
+
+// Interior cells whose internal energy is below kin_frac of the total energy:
+// for the split precision exactly the cells whose primitive variables the step
+// converts again in FP64.  One atomic per wavefront.
+template <class P>
+__global__ __launch_bounds__(256) void kinetic_count_kernel(KParams kp, Grid U,
+                                                            unsigned long long *count) {
+  unsigned n = 0;
+  for (int j = 2 + blockIdx.x; j <= kp.ny + 1; j += gridDim.x)
+    for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x)
+      n += P::kinetic_cell(kp, P::load(U, (long long)j * U.pitch + i)) ? 1u : 0u;
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+  if ((threadIdx.x & (kLanes - 1)) == 0 && n) atomicAdd(count, (unsigned long long)n);
 }
 
 // ------------------------------------------------ reference pass schedule
@@ -1347,7 +1387,7 @@ KParams make_kparams(const e2d_params &p, int precision) {
   k.dyf = (float)p.dy;
   k.cfl = p.cfl;
   k.tEnd = p.tEnd;
-  k.kin_frac = 0.1f;
+  k.kin_frac = (float)p.kin_frac;
   // exact powers of two (normal, with a normal reciprocal) in both FP64 and FP32
   auto pow2 = [](double h) {
     int e = 0;
@@ -1395,6 +1435,14 @@ hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt
   const int blocks = kp.ny < 2048 ? kp.ny : 2048;
   E2D_DISPATCH_P(kp, dt_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U,
                  invdt_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_kinetic_count(const KParams &kp, Grid U, unsigned long long *count,
+                                hipStream_t st) {
+  const int blocks = kp.ny < 2048 ? kp.ny : 2048;
+  E2D_DISPATCH_P(kp, kinetic_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U,
+                 count);
   return hipGetLastError();
 }
 

@@ -120,6 +120,9 @@ hipError_t launch_blast_energy(const KParams &kp, const e2d_params &p, Grid U, d
 hipError_t launch_boundaries(const KParams &kp, Grid U, hipStream_t st);
 hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt_bits,
                             hipStream_t st);
+// count of interior cells below the kinetic-energy threshold kp.kin_frac (added to *count)
+hipError_t launch_kinetic_count(const KParams &kp, Grid U, unsigned long long *count,
+                                hipStream_t st);
 // reference pass schedule (FP64 only)
 hipError_t launch_prim(const KParams &kp, Grid U, Grid Q, hipStream_t st);
 hipError_t launch_fluxes(const KParams &kp, Grid Q, Grid Fx, Grid Fy, double dtdx, double dtdy,

@@ -41,6 +41,7 @@ void params_defaults(e2d_params &p) {
   p.blast_pressure_out = 0.1;
   p.blast_nbins = 100;
   p.implode_pressure_in = 1.0;
+  p.kin_frac = F(0.1);
 }
 
 void params_init(e2d_params &p) {
@@ -140,6 +141,7 @@ void params_setup(const ConfigMap &cfg, e2d_params &p) {
   // extension keys; their defaults reproduce the reference
   p.honor_riemann = cfg.getBool("hydro", "honor_riemann", false) ? 1 : 0;
   p.implode_pressure_in = cfg.getFloat("implode", "pressure_in", F(1.0));
+  p.kin_frac = cfg.getFloat("hydro", "kin_frac", F(0.1));
 
   params_init(p);
 }
@@ -183,6 +185,8 @@ const char *params_check(const e2d_params &p) {
     return "slope_type must be 0, 1 or 2 (other values leave the reference's slopes unset)";
   if (p.riemannSolverType < 0 || p.riemannSolverType > 2) return "riemannSolverType must be 0, 1 or 2";
   if (!(p.cfl > 0.0)) return "cfl must be positive";
+  if (!(p.kin_frac >= 0.0 && p.kin_frac <= 1.0e30))
+    return "kin_frac must be a finite number >= 0";
   if (static_cast<long long>(p.isize) * p.jsize > (1LL << 31)) return "grid too large";
   return nullptr;
 }

@@ -255,6 +255,13 @@ class HydroRun:
         check(load_library().e2d_conserved_sums(self._h, which, out))
         return np.array(out[:])
 
+    def kinetic_cells(self, which: int) -> int:
+        """Interior cells whose internal energy is below kin_frac of the total
+        (in split precision: the cells converted again in FP64)."""
+        out = C.c_longlong(0)
+        check(load_library().e2d_kinetic_cells(self._h, which, C.byref(out)))
+        return out.value
+
     def timers(self) -> dict:
         out = (C.c_double * 5)()
         check(load_library().e2d_timers(self._h, out))

@@ -66,13 +66,18 @@ extern "C" {
  * POD mirror of HydroParams + HydroSettings + blast + shocked-bubble fields
  * (src/HydroParams.h:107-265).  Derived fields (imax, jmax, isize, jsize, dx,
  * dy, gamma6, smallp, smallpp) are filled by e2d_params_init().
- * Two extension fields, whose defaults keep the reference's behaviour:
+ * Extension fields, whose defaults keep the reference's behaviour:
  *   honor_riemann       0: HLLC always (the reference ignores `riemann`,
  *                       src/HydroRunFunctors.h:567,631); 1: use riemannSolverType
  *                       ([hydro] honor_riemann)
  *   implode_pressure_in high-state pressure of the implode problem
  *                       ([implode] pressure_in, default 1; the reference
  *                       hard-codes 1, src/HydroRunFunctors.h:1390)
+ *   kin_frac            split precision only: a cell whose FP32 pressure is
+ *                       below kin_frac * (gamma0 - 1) * E (internal energy
+ *                       below that fraction of the total) has its primitive
+ *                       variables converted again in FP64 from hi + lo
+ *                       ([hydro] kin_frac, default 0.1; 0 never converts again)
  */
 typedef struct e2d_params {
   /* [run] */
@@ -111,6 +116,8 @@ typedef struct e2d_params {
   double sb_shock_loc;
   /* [implode] */
   double implode_pressure_in;
+  /* split-precision FP64 primitive-conversion threshold */
+  double kin_frac;
 } e2d_params;
 
 typedef struct e2d_run e2d_run;       /* opaque solver handle (HydroRun) */
@@ -188,6 +195,12 @@ int e2d_conserved_sums(e2d_run *run, int which, double sums[4]);
  * boundaries, godunov, primitives, fluxes, update (the fused schedule books
  * its whole step under godunov) */
 int e2d_timers(e2d_run *run, double out[5]);
+/* Interior cells of U / U2 whose internal energy is below kin_frac of the
+ * total energy (no reference counterpart).  For the split precision these are
+ * exactly the cells whose primitive variables the step converts again in FP64
+ * (FP32 pressure below kin_frac * (gamma0 - 1) * E); for FP64 the same test in
+ * FP64.  Synchronous. */
+int e2d_kinetic_cells(e2d_run *run, int which, long long *count);
 /* E2D_SCHEDULE_*.  e2d_create picks the schedule of the params'
  * implementationVersion: 0 -> FUSED (same bits as the reference's version 0),
  * 1 -> DIRECTIONAL, 2 -> ATOMIC.  Every schedule but FUSED is FP64 only. */

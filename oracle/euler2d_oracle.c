@@ -15,7 +15,9 @@
  *   INI / ConfigMap parsing  -> the reference's own config/ sources compiled
  *                               into oracle/_ref [test_config_parity.py]
  *   HydroParams setup/init   -> float-rounding and derived-value closed forms
- *                               [test_oracle_params.py]
+ *                               [test_oracle_params.py]; smallp by the floor
+                               sound speed, smallpp by the empty-cell
+                               pressure floor
  *   computePrimitives        -> exact inverse of the textbook prim->cons map
  *                               [test_oracle_pointwise.py]
  *   slope limiter            -> minmod / MC definitions on monotone, extremum
@@ -24,11 +26,11 @@
  *                               half-step Taylor expansion of the Euler
  *                               equations [test_oracle_pointwise.py]
  *   riemann_hllc             -> Toro (2009) 10.4 HLLC written independently
- *                               from the book [test_oracle_riemann.py]
+ *                               from the book [test_oracle_pointwise.py]
  *   riemann_hll              -> Toro 10.3, two algebraically different forms
- *                               of the HLL flux [test_oracle_riemann.py]
+ *                               of the HLL flux [test_oracle_pointwise.py]
  *   riemann_approx           -> exact two-shock Riemann solution (Toro 4.2)
- *                               on shock-shock data [test_oracle_riemann.py]
+ *                               on shock-shock data [test_oracle_pointwise.py]
  *   make_boundaries          -> closed-form ghost maps brute-forced on small
  *                               grids [test_oracle_step.py]
  *   whole step / loop        -> conservation (reflecting / periodic walls),
@@ -292,6 +294,7 @@ void ora_params_defaults(ora_params *p) {
   p->blast_pressure_out = 0.1;
   p->blast_nbins = 100;
   p->implode_pressure_in = 1.0;
+  p->kin_frac = 0.1f;
 }
 
 /* HydroParams::init, HydroParams.cpp:160-190 */
@@ -384,6 +387,7 @@ void ora_params_setup(void *cfg, ora_params *p) {
   /* extensions (keys unknown to the reference; defaults keep its behaviour) */
   p->honor_riemann = ora_config_get_bool(cfg, "hydro", "honor_riemann", 0);
   p->implode_pressure_in = ora_config_get_float(cfg, "implode", "pressure_in", 1.0f);
+  p->kin_frac = ora_config_get_float(cfg, "hydro", "kin_frac", 0.1f);
 
   ora_params_init(p);
 }

@@ -62,6 +62,8 @@ typedef struct ora_params {
   double sb_shock_loc;
   /* extension: [implode] pressure_in, high-state pressure (reference hard-codes 1) */
   double implode_pressure_in;
+  /* extension: [hydro] kin_frac, split-precision threshold of the product (unused here) */
+  double kin_frac;
 } ora_params;
 
 int ora_params_size(void);

@@ -8,7 +8,7 @@ import pytest
 import oracle_lib as O
 from euler2d_kokkos_amd import (E2D_PRECISION_FP32, E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64,
                                 HydroParams, HydroRun)
-from euler2d_kokkos_amd.batch import HydroBatch, make_params
+from euler2d_kokkos_amd.batch import HydroBatch, density_l1, make_params, sweep_params
 
 pytestmark = pytest.mark.gpu
 
@@ -134,3 +134,33 @@ def test_batch_autotune_and_reinit(gpu_available):
     batch.run(10, use_graph=True)
     for i in range(len(ps)):
         np.testing.assert_array_equal(batch.state(i), a[i])
+
+
+def test_sweep_instances_accuracy(gpu_available):
+    """32 instances of the seeded 512-instance sweep, chosen to include its
+    strongest blasts (pressure ratio >= 1e3) and its 512-cell sides, run 1000
+    steps in split precision and in FP64: every instance's relative L1 density
+    error against its FP64 twin stays within 1e-4."""
+    ps = sweep_params(steps=1000)
+    ratio = [p.blast_pressure_in / p.blast_pressure_out if p.problemType == 1 else 0.0
+             for p in ps]
+    strong = sorted(range(len(ps)), key=lambda i: -ratio[i])[:12]
+    wide = [i for i in range(len(ps)) if 512 in (ps[i].nx, ps[i].ny) and i not in strong][:10]
+    rest = [i for i in range(0, len(ps), 37) if i not in strong and i not in wide]
+    pick = (strong + wide + rest)[:32]
+    sel = [ps[i] for i in pick]
+    assert min(ratio[i] for i in strong) >= 1e3
+    assert any(p.nx == 512 and p.ny == 512 for p in sel) or len(wide) >= 5
+    b64 = HydroBatch(sel, precision=E2D_PRECISION_FP64)
+    b32 = HydroBatch(sel, precision=E2D_PRECISION_FP32_SPLIT)
+    for b in (b64, b32):
+        _, active = b.run(1000)
+        assert active == 0
+    for i in range(len(sel)):
+        assert b32.status(i)[1] == b64.status(i)[1] == 1000
+    err = density_l1(b32, b64)
+    worst = int(np.argmax(err))
+    print(f"sweep subset: instances={len(sel)} max={err.max():.3e} median={np.median(err):.3e} "
+          f"worst=sweep[{pick[worst]}] {sel[worst].nx}x{sel[worst].ny} "
+          f"problem={sel[worst].problemType} ratio={ratio[pick[worst]]:.0f}", flush=True)
+    assert np.all(err <= 1e-4), err

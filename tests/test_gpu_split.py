@@ -23,6 +23,9 @@ CASES = {
     "four_quadrant_hll": dict(nx=128, ny=128, problemType=2, cfl=0.8, nStepmax=200, tEnd=1e9,
                               boundary_type_xmin=2, boundary_type_xmax=2, boundary_type_ymin=2,
                               boundary_type_ymax=2, honor_riemann=1, riemannSolverType=1),
+    # the approx (two-shock) solver in FP32
+    "blast_approx": dict(nx=96, ny=80, problemType=1, cfl=0.6, nStepmax=200, tEnd=1e9,
+                         honor_riemann=1, riemannSolverType=0, blast_radius=0.15),
     "shocked_bubble": dict(nx=192, ny=96, problemType=4, cfl=0.8, nStepmax=200, tEnd=1e9,
                            xmax=2.0, boundary_type_xmin=2, boundary_type_xmax=2,
                            sb_preshock_density=1.0, sb_preshock_pressure=1.0,
@@ -186,3 +189,77 @@ def test_wide_addressing_same_bits(gpu_available, precision, monkeypatch):
     assert (a.t, a.nStep) == (b.t, b.nStep)
     for w in (E2D_U, E2D_U2):
         np.testing.assert_array_equal(a.get_state(w), b.get_state(w))
+
+
+def _hypersonic_state(p, mach, angle=0.3):
+    """A density wave carried by a uniform flow at the given Mach number
+    (p = 1, rho = 1 + 0.3 sin(2 pi x) cos(2 pi y)): an exact translation of the
+    density field, with internal energy 1/(1 + gamma (gamma-1) M^2 / 2) of the
+    total at rho = 1."""
+    g = p.gamma0
+    x = p.xmin + (np.arange(p.isize) - 1.5) * p.dx
+    y = p.ymin + (np.arange(p.jsize) - 1.5) * p.dy
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    speed = mach * np.sqrt(g * 1.0 / 1.0)
+    u, v = speed * np.cos(angle), speed * np.sin(angle)
+    E = 1.0 / (g - 1.0) + 0.5 * rho * (u * u + v * v)
+    return np.stack([rho, E, rho * u, rho * v], axis=-1)
+
+
+def _pressure(a, gamma):
+    r, E, mx, my = (a[..., k] for k in range(4))
+    return (gamma - 1.0) * (E - 0.5 * (mx * mx + my * my) / r)
+
+
+def test_split_kinetic_fallback(gpu_available):
+    """Mach 20 uniform flow carrying a density wave (periodic): the internal
+    energy is about 0.9% of the total, so every cell falls below the default
+    kin_frac = 0.1 and takes the FP64 primitive conversion.  The branch must
+    fire (kin_frac = 0 gives other bits), keep the density within the L1 gate
+    and bring the pressure closer to the FP64 run than the FP32 conversion."""
+    base = dict(nx=128, ny=96, cfl=0.8, nStepmax=300, tEnd=1e9, boundary_type_xmin=3,
+                boundary_type_xmax=3, boundary_type_ymin=3, boundary_type_ymax=3)
+    out = {}
+    for name, prec, kf in (("fp64", E2D_PRECISION_FP64, 0.1),
+                           ("split", E2D_PRECISION_FP32_SPLIT, 0.1),
+                           ("split_nofallback", E2D_PRECISION_FP32_SPLIT, 0.0)):
+        p = O.default_params(kin_frac=kf, **base)
+        r = HydroRun(HydroParams(p.copy()), precision=prec, initialise=False)
+        u0 = _hypersonic_state(p, mach=20.0)
+        r.set_state(E2D_U, u0)
+        r.set_state(E2D_U2, u0)
+        kin0 = r.kinetic_cells(E2D_U)
+        r.run()
+        assert r.nStep == p.nStepmax
+        out[name] = (r.get_state(r.current()), kin0, r.t)
+    ref = O.interior(out["fp64"][0])
+    p64 = _pressure(ref, 1.4)
+    res = {}
+    for name in ("split", "split_nofallback"):
+        a = O.interior(out[name][0])
+        l1 = O.rel_l1(a, ref)
+        lp = np.abs(_pressure(a, 1.4) - p64).sum() / np.abs(p64).sum()
+        res[name] = (l1, lp)
+        print(f"{name}: kinetic cells at start={out[name][1]} relL1={l1.tolist()} "
+              f"pressure relL1={lp:.3e} t={out[name][2]!r}", flush=True)
+    ncell = base["nx"] * base["ny"]
+    assert out["split"][1] == ncell and out["split_nofallback"][1] == 0
+    assert not np.array_equal(out["split"][0], out["split_nofallback"][0])
+    for name in res:
+        assert res[name][0][0] <= L1_TOL, (name, res[name])
+    assert res["split"][1] < res["split_nofallback"][1], res
+
+
+def test_kin_frac_parameter(gpu_available, tmp_path):
+    """[hydro] kin_frac is read as a float like every real key; the default is
+    0.1 and a negative value is refused."""
+    from euler2d_kokkos_amd import HydroParams as HP
+    f = tmp_path / "k.ini"
+    f.write_text("[mesh]\nnx=8\nny=8\n[hydro]\nkin_frac=0.25\n")
+    assert HP.from_ini(f).kin_frac == np.float32(0.25)
+    f.write_text("[mesh]\nnx=8\nny=8\n")
+    assert HP.from_ini(f).kin_frac == np.float32(0.1)
+    p = O.default_params(nx=8, ny=8, kin_frac=-1.0)
+    with pytest.raises(E2DError):
+        HydroRun(HydroParams(p.copy()), precision=E2D_PRECISION_FP32_SPLIT)
