@@ -40,6 +40,10 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 
+# roctx ranges (rocprofv3 --marker-trace) named as the reference's profiling regions
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+ROCTX = [f"-L{ROCM / 'lib'}", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM / 'lib'}"]
+
 SOURCES = ["config.cpp", "params.cpp", "api.cpp", "io.cpp", "kernels.hip"]
 
 
@@ -66,13 +70,13 @@ def build(verbose: bool = False) -> Path:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-           *map(str, objs)]
+           *map(str, objs), *ROCTX]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
     cli_cmd = [hipcc(), *COMMON, str(CSRC / "main.cpp"), "-o", str(CLI), f"-L{PKG}",
-               "-leuler2d_hip", "-Wl,-rpath,$ORIGIN"]
+               "-leuler2d_hip", "-Wl,-rpath,$ORIGIN", *ROCTX]
     res = subprocess.run(cli_cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"CLI build failed: {' '.join(cli_cmd)}\n{res.stdout}\n{res.stderr}")

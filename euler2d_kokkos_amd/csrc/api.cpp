@@ -21,6 +21,7 @@
 #include "kernels.hpp"
 #include "params.hpp"
 #include "status.hpp"
+#include "trace.hpp"
 
 using e2d::Grid;
 
@@ -64,10 +65,12 @@ struct e2d_run {
   hipStream_t stream = nullptr;
   Grid U{}, U2{}, Q{}, Fx{}, Fy{}, Sx{}, Sy{};
   e2d::StepCtl *ctl = nullptr;
+  e2d::StepState *host_s = nullptr;  // pinned copy of ctl->s, written by the step kernels
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // the run loop's timing events
   int phase = 0;  // phase of the next run-loop launch in the control rings
-  // captured run of kLoopGraphSteps run-loop launches (for seg_rows = loop_graph_tile)
-  hipGraphExec_t loop_graph = nullptr;
-  int loop_graph_tile = 0;
+  // captured runs of kLoopGraphSteps[g] run-loop launches (for seg_rows = loop_graph_tile[g])
+  hipGraphExec_t loop_graph[2] = {nullptr, nullptr};
+  int loop_graph_tile[2] = {0, 0};
   unsigned long long *bits = nullptr;
   unsigned int *count = nullptr;
   double *partials = nullptr;
@@ -155,16 +158,28 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
     if (rc) return rc;
     // HydroRun::godunov_unsplit_impl, HydroRun.h:279-364
     if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
-    HIP_TRY(e2d::launch_boundaries(r->kp, in, r->stream));
+    {
+      e2d::Region reg("make_boundaries");
+      HIP_TRY(e2d::launch_boundaries(r->kp, in, r->stream));
+    }
     HIP_TRY(e2d::launch_copy(r->kp, in, out, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[1], r->stream));
-    HIP_TRY(e2d::launch_prim(r->kp, in, r->Q, r->stream));
+    {
+      e2d::Region reg("compute_primitives");
+      HIP_TRY(e2d::launch_prim(r->kp, in, r->Q, r->stream));
+    }
     if (timed) HIP_TRY(hipEventRecord(ev[2], r->stream));
     if (r->schedule == E2D_SCHEDULE_PASSES) {
-      HIP_TRY(e2d::launch_fluxes(r->kp, r->Q, r->Fx, r->Fy, dtdx, dtdy, r->stream));
+      e2d::Region impl("hydro_impl0");
+      {
+        e2d::Region reg("compute_fluxes");
+        HIP_TRY(e2d::launch_fluxes(r->kp, r->Q, r->Fx, r->Fy, dtdx, dtdy, r->stream));
+      }
       if (timed) HIP_TRY(hipEventRecord(ev[3], r->stream));
+      e2d::Region reg("update_hydro");
       HIP_TRY(e2d::launch_update(r->kp, out, r->Fx, r->Fy, r->stream));
     } else if (r->schedule == E2D_SCHEDULE_DIRECTIONAL) {
+      e2d::Region impl("hydro_impl1");
       HIP_TRY(e2d::launch_slopes(r->kp, r->Q, r->Sx, r->Sy, r->stream));
       HIP_TRY(e2d::launch_trace_fluxes(r->kp, 0, r->Q, r->Sx, r->Sy, r->Fx, dtdx, dtdy,
                                        r->stream));
@@ -173,11 +188,14 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
                                        r->stream));
       HIP_TRY(e2d::launch_update_dir(r->kp, 1, out, r->Fy, r->stream));
     } else {
+      e2d::Region impl("hydro_impl2");
       HIP_TRY(e2d::launch_fluxes_atomic(r->kp, r->Q, out, dtdx, dtdy, r->stream));
     }
     if (timed) HIP_TRY(hipEventRecord(ev[4], r->stream));
   } else {
     if (timed) HIP_TRY(hipEventRecord(ev[0], r->stream));
+    // implementation version 0 as one fused launch (boundaries to update)
+    e2d::Region impl("hydro_impl0");
     HIP_TRY(e2d::launch_step_fused(r->kp, in, out, dt, nullptr, 0, r->seg_rows, r->stream));
     if (timed) HIP_TRY(hipEventRecord(ev[4], r->stream));
   }
@@ -210,63 +228,77 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
 // the device from the current state.
 // Phase 0 of the control rings: the state before the loop's first step and
 // the CFL rate of its input; every other entry cleared.
+// One launch: at nStep == 0 it also fills the ghost cells of U and U2
+// (main.cpp:90-91).
 int loop_begin(e2d_run *r, double t, int nStep) {
   r->phase = 0;
-  HIP_TRY(hipMemsetAsync(r->ctl, 0, sizeof(e2d::StepCtl), r->stream));
-  e2d::StepState s{};
-  s.t = t;
-  s.nstep = nStep;
-  HIP_TRY(hipMemcpyAsync(&r->ctl->s[0], &s, sizeof(s), hipMemcpyHostToDevice, r->stream));
-  HIP_TRY(e2d::launch_dt_reduce(r->kp, (nStep % 2 == 0) ? r->U : r->U2, &r->ctl->rate[0][0],
-                                r->stream));
+  const bool even = nStep % 2 == 0;
+  HIP_TRY(e2d::launch_loop_begin(r->kp, even ? r->U : r->U2, even ? r->U2 : r->U, nStep == 0,
+                                 r->ctl, t, nStep, r->partials, r->stream));
   return E2D_OK;
 }
 
-// Run-loop launches per captured graph: a multiple of 6, so that a replay that
-// starts at ring phase 0 on an even step ends there (phase ring 3, array parity 2).
-constexpr int kLoopGraphSteps = 30;
+// Run-loop launches per captured graph: multiples of 6, so that a replay that
+// starts at ring phase 0 on an even step ends there (phase ring 3, array
+// parity 2).  The long one carries long loops; the short one lets a budget
+// below 30 steps replay graphs too.
+constexpr int kLoopGraphSteps[2] = {30, 6};
 
-int loop_graph(e2d_run *r) {
-  if (r->loop_graph && r->loop_graph_tile == r->seg_rows) return E2D_OK;
-  if (r->loop_graph) (void)hipGraphExecDestroy(r->loop_graph);
-  r->loop_graph = nullptr;
-  hipGraph_t g = nullptr;
+int loop_graph(e2d_run *r, int g) {
+  if (r->loop_graph[g] && r->loop_graph_tile[g] == r->seg_rows) return E2D_OK;
+  if (r->loop_graph[g]) (void)hipGraphExecDestroy(r->loop_graph[g]);
+  r->loop_graph[g] = nullptr;
+  hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < kLoopGraphSteps; ++k) {
+  for (int k = 0; k < kLoopGraphSteps[g]; ++k) {
     const Grid &in = (k % 2 == 0) ? r->U : r->U2;
     const Grid &out = (k % 2 == 0) ? r->U2 : r->U;
     hipError_t e = e2d::launch_step_fused(r->kp, in, out, 0.0, r->ctl, k % 3, r->seg_rows,
                                           r->stream);
     if (e != hipSuccess) {
-      (void)hipStreamEndCapture(r->stream, &g);
-      if (g) (void)hipGraphDestroy(g);
+      (void)hipStreamEndCapture(r->stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
       return hip_fail(e, "launch_step_fused (capture)");
     }
   }
-  HIP_TRY(hipStreamEndCapture(r->stream, &g));
-  hipError_t e = hipGraphInstantiate(&r->loop_graph, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
+  HIP_TRY(hipStreamEndCapture(r->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&r->loop_graph[g], graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
-    r->loop_graph = nullptr;
+    r->loop_graph[g] = nullptr;
     return hip_fail(e, "hipGraphInstantiate");
   }
-  r->loop_graph_tile = r->seg_rows;
+  r->loop_graph_tile[g] = r->seg_rows;
   return E2D_OK;
 }
 
-// Queues `count` run-loop launches from step nStep0 on: captured runs of
-// kLoopGraphSteps where the ring phase is 0 and the step even, single launches
+// Captures, ahead of a timed region, the graphs a loop of `count` launches
+// will replay (capture and instantiation are host work of a few ms).
+int loop_graphs_for(e2d_run *r, int count) {
+  for (int g = 0; g < 2; ++g)
+    if (count >= kLoopGraphSteps[g]) {
+      int rc = loop_graph(r, g);
+      if (rc) return rc;
+    }
+  return E2D_OK;
+}
+
+// Queues `count` run-loop launches from step nStep0 on: captured runs of 30
+// or 6 launches where the ring phase is 0 and the step even, single launches
 // elsewhere.
 int loop_steps(e2d_run *r, int nStep0, int count) {
   int k = 0;
   while (k < count) {
     const int n = nStep0 + k;
-    if (count - k >= kLoopGraphSteps && r->phase == 0 && n % 2 == 0) {
-      int rc = loop_graph(r);
-      if (rc) return rc;
-      HIP_TRY(hipGraphLaunch(r->loop_graph, r->stream));
-      k += kLoopGraphSteps;
-      continue;
+    if (r->phase == 0 && n % 2 == 0) {
+      const int g = count - k >= kLoopGraphSteps[0] ? 0 : (count - k >= kLoopGraphSteps[1] ? 1 : -1);
+      if (g >= 0) {
+        int rc = loop_graph(r, g);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(r->loop_graph[g], r->stream));
+        k += kLoopGraphSteps[g];
+        continue;
+      }
     }
     Grid in = (n % 2 == 0) ? r->U : r->U2;
     Grid out = (n % 2 == 0) ? r->U2 : r->U;
@@ -293,11 +325,14 @@ LoopView loop_view(const e2d_params &p, const e2d::StepState &s) {
   return v;
 }
 
+// The kernels also store each ring entry in pinned host memory, so reading
+// the loop state needs only the synchronisation.
 int loop_read(e2d_run *r, LoopView &v) {
-  e2d::StepState s{};
-  HIP_TRY(hipMemcpyAsync(&s, &r->ctl->s[r->phase], sizeof(s), hipMemcpyDeviceToHost,
-                         r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
+  const volatile e2d::StepState &hs = r->host_s[r->phase];
+  e2d::StepState s{};
+  s.t = hs.t;
+  s.nstep = hs.nstep;
   v = loop_view(r->p, s);
   return E2D_OK;
 }
@@ -305,6 +340,7 @@ int loop_read(e2d_run *r, LoopView &v) {
 int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elapsed_ms,
                   int *executed) {
   if (!t || !nStep) return fail(E2D_ERR_ARG, "null t / nStep");
+  e2d::Region reg("main_loop");
   if (r->schedule != E2D_SCHEDULE_FUSED) {
     // reference structure: host dt per step, exactly main.cpp:100-145
     const int n0 = *nStep;
@@ -340,17 +376,14 @@ int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elap
     return E2D_OK;
   }
 
-  if (*nStep == 0) {
-    // main.cpp:90-91 (only the ghost cells are affected)
-    HIP_TRY(e2d::launch_boundaries(r->kp, r->U, r->stream));
-    HIP_TRY(e2d::launch_boundaries(r->kp, r->U2, r->stream));
-  }
   const int n0 = *nStep;
-  int rc = loop_begin(r, *t, *nStep);
+  // graphs first: their capture is host work that would otherwise fall inside
+  // the timed region
+  int rc = loop_graphs_for(r, max_steps > 0 ? std::min(max_steps, r->p.nStepmax - n0) : kChunk);
   if (rc) return rc;
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  rc = loop_begin(r, *t, *nStep);
+  if (rc) return rc;
+  hipEvent_t e0 = r->ev0, e1 = r->ev1;
   HIP_TRY(hipEventRecord(e0, r->stream));
   LoopView c;
   int launched = 0;
@@ -364,6 +397,9 @@ int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elap
     rc = loop_steps(r, n0 + launched, todo);
     if (rc) return rc;
     launched += todo;
+    // an explicit budget is queued whole: the end event follows the last
+    // launch directly, with no host round trip in between
+    if (max_steps > 0 && launched >= max_steps) break;
     rc = loop_read(r, c);
     if (rc) return rc;
     if (!c.active) break;
@@ -373,8 +409,6 @@ int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elap
   if (rc) return rc;
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   r->timers[1] += ms * 1e-3;
   *t = c.t;
   *nStep = c.nstep;
@@ -511,7 +545,12 @@ int e2d_device_info(char *name, int name_len, int *num_cu) {
   e = hipGetDeviceProperties(&prop, 0);
   if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
   if (name && name_len > 0) {
-    std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    // the marketing name can be empty (a device seen without its product
+    // table); the architecture name is always there
+    if (prop.name[0])
+      std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    else
+      std::snprintf(name, (size_t)name_len, "%s", prop.gcnArchName);
   }
   if (num_cu) *num_cu = prop.multiProcessorCount;
   return E2D_OK;
@@ -657,6 +696,11 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
     if (!rc2) rc2 = alloc_dev(r, &ptr, 4 * sizeof(double));
     if (!rc2) r->sums4 = static_cast<double *>(ptr);
     if (rc2) return rc2;
+    HIP_TRY(hipEventCreate(&r->ev0));
+    HIP_TRY(hipEventCreate(&r->ev1));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&r->host_s), 3 * sizeof(e2d::StepState),
+                          hipHostMallocDefault));
+    std::memset(r->host_s, 0, 3 * sizeof(e2d::StepState));
     // 32-bit row offsets in the step kernels need planes of at most 4 GiB;
     // E2D_WIDE_ADDRESSING=1 selects the 64-bit form for any size
     const char *wide = std::getenv("E2D_WIDE_ADDRESSING");
@@ -669,6 +713,8 @@ int e2d_create(const e2d_params *p, int precision, e2d_run **out) {
     HIP_TRY(hipMemsetAsync(r->U.p, 0, bytes, r->stream));
     HIP_TRY(hipMemsetAsync(r->U2.p, 0, bytes, r->stream));
     HIP_TRY(hipMemsetAsync(r->ctl, 0, sizeof(e2d::StepCtl), r->stream));
+    HIP_TRY(hipMemcpyAsync(&r->ctl->host_s, &r->host_s, sizeof(r->host_s), hipMemcpyHostToDevice,
+                           r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
     return E2D_OK;
     }();
@@ -686,7 +732,11 @@ int e2d_destroy(e2d_run *run) {
   if (!run) return fail(E2D_ERR_ARG, "null e2d_run handle");
   (void)hipSetDevice(run->device);
   if (run->stream) (void)hipStreamSynchronize(run->stream);
-  if (run->loop_graph) (void)hipGraphExecDestroy(run->loop_graph);
+  for (hipGraphExec_t g : run->loop_graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  if (run->ev0) (void)hipEventDestroy(run->ev0);
+  if (run->ev1) (void)hipEventDestroy(run->ev1);
+  if (run->host_s) (void)hipHostFree(run->host_s);
   for (void *ptr : run->allocs) (void)hipFree(ptr);
   if (run->stream) (void)hipStreamDestroy(run->stream);
   delete run;
@@ -728,6 +778,7 @@ int e2d_compute_dt(e2d_run *r, int useU, double *dt) {
   int rc = check_run(r);
   if (rc) return rc;
   if (!dt) return fail(E2D_ERR_ARG, "null dt");
+  e2d::Region reg("compute_dt");
   HIP_TRY(hipMemsetAsync(r->bits, 0, sizeof(unsigned long long), r->stream));
   HIP_TRY(e2d::launch_dt_reduce(r->kp, useU == 0 ? r->U : r->U2, r->bits, r->stream));
   unsigned long long bits = 0;
@@ -743,6 +794,7 @@ int e2d_make_boundaries(e2d_run *r, int which) {
   int rc = check_run(r);
   if (rc) return rc;
   if (which != E2D_U && which != E2D_U2) return fail(E2D_ERR_ARG, "which must be E2D_U or E2D_U2");
+  e2d::Region reg("make_boundaries");
   HIP_TRY(e2d::launch_boundaries(r->kp, state(r, which), r->stream));
   return E2D_OK;
 }

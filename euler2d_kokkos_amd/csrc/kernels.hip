@@ -534,6 +534,108 @@ __global__ __launch_bounds__(256) void dt_reduce_kernel(KParams kp, Grid U,
   }
 }
 
+// The copy of a ring entry in host memory (when the control block has one):
+// system-scope stores reach host memory past the device caches, so the host
+// sees them once the launch has completed.
+__device__ __forceinline__ void mirror_state(StepState *hs, int ph, const StepState &n) {
+  if (!hs) return;
+  unsigned long long *w = reinterpret_cast<unsigned long long *>(hs + ph);
+  __hip_atomic_store(w, (unsigned long long)__double_as_longlong(n.t), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(w + 1, (unsigned long long)(unsigned)n.nstep, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Start of a run loop in one launch (main.cpp:90-91 and the first compute_dt,
+// HydroRun.h:227-251): block b folds the CFL rate of interior rows b, b + G,
+// ... into partials[b]; with `bc` the grid's threads also fill the ghost cells
+// of `in` and `other` (make_boundaries of both arrays before step 0: the ghost
+// cells written are disjoint from the interior cells read).  The last block
+// to arrive (atomicInc wraps the arrival counter back to 0 for the next use)
+// takes the max of the partials and sets ring phase 0: the state before the
+// step and its rate in s[0] / rate[0], rate[1] cleared for launch 0's fold.
+// Launch 0 clears rate[2] itself.
+
+template <class P>
+__global__ __launch_bounds__(256) void loop_begin_kernel(KParams kp, Grid in, Grid other, int bc,
+                                                         StepCtl *ctl, double t, int nstep,
+                                                         double *partials,
+                                                         unsigned *arrivals) {
+  using T = typename P::T;
+  if (bc) {  // one thread per ghost cell of each array, as boundaries_kernel
+    const int nghost = 4 * kp.isize + 4 * kp.ny;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < 2 * nghost;
+         g += gridDim.x * blockDim.x) {
+      const Grid &U = g < nghost ? in : other;
+      const int t0 = g < nghost ? g : g - nghost;
+      int i, j;
+      if (t0 < 4 * kp.isize) {
+        const int r = t0 / kp.isize;
+        i = t0 - r * kp.isize;
+        j = r < 2 ? r : kp.ny + r;
+      } else {
+        const int t2 = t0 - 4 * kp.isize;
+        j = 2 + (t2 >> 2);
+        const int c = t2 & 3;
+        i = c < 2 ? c : kp.nx + c;
+      }
+      bool fx, fy;
+      const int i0 = ghost_map(i, kp.nx, kp.bcxmin, kp.bcxmax, fx);
+      const int j0 = ghost_map(j, kp.ny, kp.bcymin, kp.bcymax, fy);
+      typename P::Cell c = P::load(U, (long long)j0 * U.pitch + i0);
+      if (fx) P::negate(c, IU);
+      if (fy) P::negate(c, IV);
+      P::store(U, (long long)j * U.pitch + i, c);
+    }
+  }
+  T m = T(0.0);
+  for (int j = 2 + blockIdx.x; j <= kp.ny + 1; j += gridDim.x)
+    for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x) {
+      Q4T<T> q;
+      P::prim(kp, P::load(in, (long long)j * in.pitch + i), q);
+      m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, P::rate(kp)));
+    }
+  __shared__ double wm[256 / kLanes];
+  __shared__ bool last;
+  const double md = (double)wave_max(m);
+  if ((threadIdx.x & (kLanes - 1)) == 0) wm[threadIdx.x / kLanes] = md;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kLanes); w++)
+      if (wm[w] == wm[w] && wm[w] > b) b = wm[w];
+    partials[blockIdx.x] = b;
+    __threadfence();
+    last = atomicInc(arrivals, gridDim.x - 1) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // the last block: max over the partials (every block's store is visible)
+  double b = 0.0;
+  for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
+    const double x = __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b = x > b ? x : b;
+  }
+  b = wave_max(b);
+  if ((threadIdx.x & (kLanes - 1)) == 0) wm[threadIdx.x / kLanes] = b;
+  __syncthreads();
+  if (threadIdx.x < kRateSlots) {
+    double mx = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kLanes); w++) mx = wm[w] > mx ? wm[w] : mx;
+    ctl->rate[0][threadIdx.x * kSlotStride] = threadIdx.x == 0 ? dbits(mx) : 0ull;
+    ctl->rate[1][threadIdx.x * kSlotStride] = 0ull;
+  }
+  if (threadIdx.x == 0) {
+    StepState s0;
+    s0.t = t;
+    s0.nstep = nstep;
+    s0.pad = 0;
+    ctl->s[0] = s0;
+    mirror_state(ctl->host_s, 0, s0);
+  }
+}
+
 // Interior cells whose internal energy is below kin_frac of the total energy:
 // for the split precision exactly the cells whose primitive variables the step
 // converts again in FP64.  One atomic per wavefront.
@@ -1058,6 +1160,7 @@ __device__ __forceinline__ void loop_publish(StepCtl *ctl, int ph, const LoopPos
     n.nstep = L.active ? L.nstep + 1 : L.nstep;
     n.pad = 0;
     ctl->s[p1] = n;
+    mirror_state(ctl->host_s, p1, n);
   }
 }
 
@@ -1435,6 +1538,13 @@ hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt
   const int blocks = kp.ny < 2048 ? kp.ny : 2048;
   E2D_DISPATCH_P(kp, dt_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U,
                  invdt_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_loop_begin(const KParams &kp, Grid in, Grid other, int bc, StepCtl *ctl,
+                             double t, int nstep, double *partials, hipStream_t st) {
+  E2D_DISPATCH_P(kp, loop_begin_kernel, dim3(kBeginBlocks), dim3(256), 0, st, kp, in, other, bc,
+                 ctl, t, nstep, partials, &ctl->arrivals);
   return hipGetLastError();
 }
 

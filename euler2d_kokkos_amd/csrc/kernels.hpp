@@ -59,11 +59,18 @@ struct StepState {
   int nstep;  // steps executed before it
   int pad;
 };
+constexpr int kBeginBlocks = 512;  // blocks (and partials) of loop_begin_kernel
 constexpr int kRateSlots = 16;   // wavefront w folds into slot w % kRateSlots
 constexpr int kSlotStride = 16;  // one 128-byte line per slot
 struct StepCtl {
   StepState s[3];
-  unsigned long long pad[10];
+  // host-visible copy of s (pinned host memory, or null): the step's writer
+  // stores there too, so that the host reads t and nStep after a stream
+  // synchronisation without a copy command
+  StepState *host_s;
+  unsigned arrivals;  // loop_begin_kernel's block counter, 0 between launches
+  unsigned pad32;
+  unsigned long long pad[8];
   // max over the slots of (c+|u|)/dx+(c+|v|)/dy as IEEE bits (0 = none yet)
   unsigned long long rate[3][kRateSlots * kSlotStride];
 };
@@ -120,6 +127,11 @@ hipError_t launch_blast_energy(const KParams &kp, const e2d_params &p, Grid U, d
 hipError_t launch_boundaries(const KParams &kp, Grid U, hipStream_t st);
 hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt_bits,
                             hipStream_t st);
+// start of a run loop from (t, nstep): CFL rate of `in` into ring phase 0 of
+// *ctl, with `bc` also the ghost cells of `in` and `other`; partials holds
+// kBeginBlocks doubles
+hipError_t launch_loop_begin(const KParams &kp, Grid in, Grid other, int bc, StepCtl *ctl,
+                             double t, int nstep, double *partials, hipStream_t st);
 // count of interior cells below the kinetic-energy threshold kp.kin_frac (added to *count)
 hipError_t launch_kinetic_count(const KParams &kp, Grid U, unsigned long long *count,
                                 hipStream_t st);

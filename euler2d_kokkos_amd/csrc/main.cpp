@@ -12,10 +12,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <optional>
 #include <string>
 #include <vector>
 
 #include "euler2d_hip.h"
+#include "trace.hpp"
 
 static int die(const char *what) {
   std::fprintf(stderr, "%s: %s\n", what, e2d_last_error());
@@ -84,6 +86,7 @@ int main(int argc, char *argv[]) {
   double t_io = 0.0;
   // HydroRun::saveData (HydroRun.h:486-497) of the current state array
   auto save = [&](int step, double t) -> int {
+    e2d::Region reg("output");
     std::printf("Output results at time t=%g step %d\n", t, step);
     const auto s0 = std::chrono::steady_clock::now();
     if (params.ioVTK &&
@@ -102,6 +105,8 @@ int main(int argc, char *argv[]) {
   int nStep = 0;
   std::printf("Start computation....\n");
   const auto t0 = std::chrono::steady_clock::now();
+  std::optional<e2d::Region> main_loop;
+  main_loop.emplace("main_loop");
   // chunks end at every 10th step (the reference's progress print,
   // main.cpp:103-106) and at every output step
   while (t < params.tEnd && nStep < params.nStepmax) {
@@ -113,6 +118,7 @@ int main(int argc, char *argv[]) {
   }
   if (output && save(nStep, t)) return die("output");  // last state (main.cpp:148-161)
   if (e2d_synchronize(run)) return die("sync");
+  main_loop.reset();
   const double t_tot =
     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 

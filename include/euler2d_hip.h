@@ -220,11 +220,14 @@ int e2d_autotune(e2d_run *run, const int *candidates, int ncand, int steps, int 
 int e2d_synchronize(e2d_run *run);
 /* Device-side timing of `nsteps` loop iterations (e2d_run_loop semantics) with
  * HIP events on the handle's stream; returns the elapsed milliseconds and the
- * number of steps actually executed.  Used by benchmarks.  Runs of 30 or more
- * steps replay a captured graph of 30 launches; the first such run on a handle
- * (and the first after a tile change) captures and instantiates it, which
- * takes a few milliseconds: warm up with e2d_run_loop over 30 or more steps
- * before timing. */
+ * number of steps actually executed.  Used by benchmarks.  The loop's step
+ * launches are replayed from captured graphs of 30 and 6 launches where the
+ * budget allows; a graph missing for the current tile is captured and
+ * instantiated (a few milliseconds of host work) before the start event is
+ * recorded, so it never falls inside the timed region.  The loop start is one
+ * launch (the first step's CFL rate, and at nStep == 0 the ghost cells of U
+ * and U2), and the end needs no copy command: the kernels also store the loop
+ * state in pinned host memory. */
 int e2d_time_loop(e2d_run *run, int nsteps, double *elapsed_ms, int *executed, double *t,
                   int *nStep);
 

@@ -16,8 +16,8 @@
  *                               into oracle/_ref [test_config_parity.py]
  *   HydroParams setup/init   -> float-rounding and derived-value closed forms
  *                               [test_oracle_params.py]; smallp by the floor
-                               sound speed, smallpp by the empty-cell
-                               pressure floor
+ *                               sound speed, smallpp by the empty-cell
+ *                               pressure floor
  *   computePrimitives        -> exact inverse of the textbook prim->cons map
  *                               [test_oracle_pointwise.py]
  *   slope limiter            -> minmod / MC definitions on monotone, extremum
