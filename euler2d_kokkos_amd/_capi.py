@@ -109,6 +109,9 @@ _PROTOTYPES = {
     "e2d_make_boundaries": (C.c_int, [C.c_void_p, C.c_int]),
     "e2d_godunov_unsplit": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "e2d_run_loop": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "e2d_run_loop_noalloc": (C.c_int, [C.POINTER(E2DParams), C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_longlong, C.c_longlong, C.c_int, C.c_int,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p]),
     "e2d_copy_state": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "e2d_upload_state": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "e2d_copy_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),

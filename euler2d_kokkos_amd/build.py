@@ -44,7 +44,7 @@ COMMON = [
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ROCTX = [f"-L{ROCM / 'lib'}", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM / 'lib'}"]
 
-SOURCES = ["config.cpp", "params.cpp", "api.cpp", "io.cpp", "kernels.hip"]
+SOURCES = ["config.cpp", "params.cpp", "api.cpp", "io.cpp", "kernels.hip", "noalloc.hip"]
 
 
 def hipcc() -> str:

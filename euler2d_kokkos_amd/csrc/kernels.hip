@@ -1388,6 +1388,8 @@ __global__ void sums_final_kernel(const double *partials, int nblocks, double *o
   if (threadIdx.x < 4) out4[threadIdx.x] = red[threadIdx.x][0];
 }
 
+__global__ void fill_u32_kernel(unsigned *p, unsigned v) { *p = v; }
+
 // Plain device copy (Kokkos::deep_copy of a whole state array, HydroRun.h:214),
 // 16 bytes per lane per access.
 __global__ void copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
@@ -1538,6 +1540,11 @@ hipError_t launch_dt_reduce(const KParams &kp, Grid U, unsigned long long *invdt
   const int blocks = kp.ny < 2048 ? kp.ny : 2048;
   E2D_DISPATCH_P(kp, dt_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, kp, U,
                  invdt_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned *p, unsigned v, hipStream_t st) {
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, st, p, v);
   return hipGetLastError();
 }
 

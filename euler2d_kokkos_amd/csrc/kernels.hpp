@@ -165,6 +165,8 @@ hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int
 // per-bin cell counts and density sums of the radial profile (nbins entries each)
 hipError_t launch_radial_profile(const KParams &kp, Grid U, const RadialGeom &geo,
                                  unsigned long long *counts, double *sums, hipStream_t st);
+// *p = v, one thread (a kernel, so that it may be captured in a graph)
+hipError_t launch_fill_u32(unsigned *p, unsigned v, hipStream_t st);
 // whole-array copy (all planes, ghosts and padding included)
 hipError_t launch_copy(const KParams &kp, Grid src, Grid dst, hipStream_t st);
 

@@ -183,6 +183,36 @@ int e2d_godunov_unsplit(e2d_run *run, int nStep, double dt);
  * cap).  dt, the tEnd clamp and t += dt stay on the device (no host round
  * trip per step); returns the new t and nStep. */
 int e2d_run_loop(e2d_run *run, int max_steps, double *t, int *nStep);
+
+/* Bytes of the static device array that holds e2d_run_loop_noalloc's scratch
+ * (one copy per GPU, kept until the process ends). */
+#define E2D_NOALLOC_SCRATCH_BYTES 16384
+/* e2d_run_loop on caller-owned state arrays, allocating nothing.  No
+ * counterpart in the reference: this is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or other
+ * allocating routine; the loop's scratch (control rings, loop-start partial
+ * maxima, arrival counter) lives in a static device array of
+ * E2D_NOALLOC_SCRATCH_BYTES per GPU, whose size does not depend on the grid.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (two host threads, or two streams) race, so serialise them.
+ *   U, U2     the two state arrays (device memory, distinct), each 4 planes of
+ *             isize x jsize cells (ghosts included) in this library's device
+ *             layout for `precision`: element (v, j, i) at index
+ *             v * plane + j * pitch + i, an element being a double (FP64), a
+ *             (hi, lo) pair of floats (split, value hi + lo) or a float
+ *             (FP32); aligned to the element size
+ *   pitch     elements between rows (>= isize); plane: elements between
+ *             planes (>= pitch * jsize)
+ *   seg_rows  the step kernel's tile (0: 16; results do not depend on it)
+ *   stream    a hipStream_t (NULL: the default stream)
+ * Otherwise as e2d_run_loop: *t and *nStep hold the loop state on entry and
+ * on return; at *nStep == 0 the ghost cells of both arrays are filled first;
+ * runs until tEnd, nStepmax or max_steps iterations (<= 0: no cap).  The
+ * step's input is U for even nStep, else U2.  Synchronous. */
+int e2d_run_loop_noalloc(const e2d_params *params, int precision, void *U, void *U2,
+                         long long pitch, long long plane, int seg_rows, int max_steps,
+                         double *t, int *nStep, void *stream);
 /* device <-> host copies of U / U2 in one of the E2D_LAYOUT_* layouts */
 int e2d_copy_state(e2d_run *run, int which, double *host, int layout);
 int e2d_upload_state(e2d_run *run, int which, const double *host, int layout);
