@@ -273,10 +273,12 @@ int loop_graph(e2d_run *r, int g) {
 }
 
 // Captures, ahead of a timed region, the graphs a loop of `count` launches
-// will replay (capture and instantiation are host work of a few ms).
+// will replay (capture and instantiation are host work of about a
+// millisecond).  The short graph is captured by every loop, however short,
+// so that a warm-up of a few steps prepares a later budget of 6 or more.
 int loop_graphs_for(e2d_run *r, int count) {
   for (int g = 0; g < 2; ++g)
-    if (count >= kLoopGraphSteps[g]) {
+    if (g == 1 || count >= kLoopGraphSteps[g]) {
       int rc = loop_graph(r, g);
       if (rc) return rc;
     }
