@@ -554,13 +554,16 @@ __device__ __forceinline__ void mirror_state(StepState *hs, int ph, const StepSt
 // to arrive (atomicInc wraps the arrival counter back to 0 for the next use)
 // takes the max of the partials and sets ring phase 0: the state before the
 // step and its rate in s[0] / rate[0], rate[1] cleared for launch 0's fold.
-// Launch 0 clears rate[2] itself.
+// Launch 0 clears rate[2] itself.  Blocks of 16 wavefronts keep the loads of
+// many rows in flight (with 4 the kernel waited on load latency).
+constexpr int kBeginThreads = 1024;
 
 template <class P>
-__global__ __launch_bounds__(256) void loop_begin_kernel(KParams kp, Grid in, Grid other, int bc,
-                                                         StepCtl *ctl, double t, int nstep,
-                                                         double *partials,
-                                                         unsigned *arrivals) {
+__global__ __launch_bounds__(kBeginThreads) void loop_begin_kernel(KParams kp, Grid in,
+                                                                   Grid other, int bc,
+                                                                   StepCtl *ctl, double t,
+                                                                   int nstep, double *partials,
+                                                                   unsigned *arrivals) {
   using T = typename P::T;
   if (bc) {  // one thread per ghost cell of each array, as boundaries_kernel
     const int nghost = 4 * kp.isize + 4 * kp.ny;
@@ -588,14 +591,27 @@ __global__ __launch_bounds__(256) void loop_begin_kernel(KParams kp, Grid in, Gr
       P::store(U, (long long)j * U.pitch + i, c);
     }
   }
+  // rows b, b + G, b + 2G, b + 3G loaded together, then converted
+  constexpr int kRows = 4;
   T m = T(0.0);
-  for (int j = 2 + blockIdx.x; j <= kp.ny + 1; j += gridDim.x)
+  for (int j0 = 2 + blockIdx.x; j0 <= kp.ny + 1; j0 += kRows * gridDim.x)
     for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x) {
-      Q4T<T> q;
-      P::prim(kp, P::load(in, (long long)j * in.pitch + i), q);
-      m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, P::rate(kp)));
+      typename P::Cell c[kRows];
+#pragma unroll
+      for (int u = 0; u < kRows; u++) {
+        const int j = j0 + u * (int)gridDim.x;
+        if (j <= kp.ny + 1) c[u] = P::load(in, (long long)j * in.pitch + i);
+      }
+#pragma unroll
+      for (int u = 0; u < kRows; u++) {
+        if (j0 + u * (int)gridDim.x <= kp.ny + 1) {
+          Q4T<T> q;
+          P::prim(kp, c[u], q);
+          m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, P::rate(kp)));
+        }
+      }
     }
-  __shared__ double wm[256 / kLanes];
+  __shared__ double wm[kBeginThreads / kLanes];
   __shared__ bool last;
   const double md = (double)wave_max(m);
   if ((threadIdx.x & (kLanes - 1)) == 0) wm[threadIdx.x / kLanes] = md;
@@ -1550,8 +1566,8 @@ hipError_t launch_fill_u32(unsigned *p, unsigned v, hipStream_t st) {
 
 hipError_t launch_loop_begin(const KParams &kp, Grid in, Grid other, int bc, StepCtl *ctl,
                              double t, int nstep, double *partials, hipStream_t st) {
-  E2D_DISPATCH_P(kp, loop_begin_kernel, dim3(kBeginBlocks), dim3(256), 0, st, kp, in, other, bc,
-                 ctl, t, nstep, partials, &ctl->arrivals);
+  E2D_DISPATCH_P(kp, loop_begin_kernel, dim3(kBeginBlocks), dim3(kBeginThreads), 0, st, kp, in,
+                 other, bc, ctl, t, nstep, partials, &ctl->arrivals);
   return hipGetLastError();
 }
 
