@@ -7,10 +7,13 @@ raises, so nothing silently runs on the CPU.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "libeuler2d_hip.so"
+# E2D_LIBRARY names another build of the library (a variant built with other
+# compile-time options); by default the in-tree build
+LIB_PATH = Path(os.environ.get("E2D_LIBRARY") or PKG / "libeuler2d_hip.so")
 
 E2D_OK = 0
 E2D_ERR_ARG = -1
