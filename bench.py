@@ -16,7 +16,8 @@ the algorithmic 64 bytes per cell-update (read 4 doubles, write 4 doubles).
 `fp32_split` repeats the run in split precision (hi+lo FP32 state) and reports
 its distance from the FP64 result; `batched` times the seeded 512-instance
 ragged sweep (sizes 64..512 per side, FP32 split, 1000 steps each) advanced by
-one batched launch per step, and reports every instance's L1 density error
+one persistent launch (and, for comparison, by one launch per step), and
+reports every instance's L1 density error
 against the same sweep run in FP64; `configs` times the other decks (implode
 256x256, four-quadrant 8192x8192, shocked bubble 16384x8192) over a bounded
 number of steps in both precisions, with the split run's L1 error and drift.
@@ -127,31 +128,43 @@ def parse_args():
 
 def batch_line(steps: int) -> dict:
     """The 512-instance ragged sweep (FP32 split, HLLC, 1000 steps each) in one
-    batched launch per step, replayed from hipGraphs; tile autotuned first.
-    The same sweep then runs in FP64 (also timed), and every instance's
-    relative L1 density error against its FP64 twin is reported."""
+    persistent launch (per-XCD work queues); tile autotuned first.  The same
+    sweep then runs with one launch per step, replayed from hipGraphs (tile
+    autotuned for that mode), and both again in FP64; every instance's relative
+    L1 density error against its FP64 twin is reported."""
     import numpy as np
 
-    from euler2d_kokkos_amd import E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64
+    from euler2d_kokkos_amd import (E2D_BATCH_LAUNCH_PERSISTENT, E2D_BATCH_LAUNCH_STEPS,
+                                    E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64)
     from euler2d_kokkos_amd.batch import HydroBatch, density_l1, sweep_params
 
     ps = sweep_params(steps=steps)
-    cand = (8, 12, 16, 24, 32, 48, 64)
+    cand = (12, 16, 24, 32, 48, 64, 96)
+
+    def timed(b, mode):
+        b.set_launch(mode)
+        b.run(60)  # the engine clock settles before the candidates are timed
+        # candidates differ by a few percent, which runs of 20 steps do not
+        # resolve; 60 + 2 x 7 x 62 steps stay within the sweep's 1000
+        tile, _ = b.autotune(candidates=cand, steps=60)
+        b.init()
+        t0 = time.perf_counter()
+        ms, active = b.run(steps, use_graph=True)
+        wall = time.perf_counter() - t0
+        cell_updates = sum(p.nx * p.ny * b.status(i)[1] for i, p in enumerate(ps))
+        return {"value": round(cell_updates / wall / 1e6, 2), "tile": tile,
+                "device_ms": round(ms, 3),
+                "instance_steps_done": sum(b.status(i)[1] for i in range(len(ps))),
+                "still_active": active}
+
     out = {}
     runs = {}
     for name, prec in (("fp32_split", E2D_PRECISION_FP32_SPLIT), ("fp64", E2D_PRECISION_FP64)):
         b = HydroBatch(ps, precision=prec)
-        tile, _ = b.autotune(candidates=cand, steps=5)
-        b.init()
         info = b.info()
-        t0 = time.perf_counter()
-        ms, active = b.run(steps, use_graph=True)
-        wall = time.perf_counter() - t0
-        done = sum(b.status(i)[1] for i in range(info["count"]))
-        cell_updates = sum(p.nx * p.ny * b.status(i)[1] for i, p in enumerate(ps))
-        out[name] = {"value": round(cell_updates / wall / 1e6, 2), "tile": tile,
-                     "device_ms": round(ms, 3), "instance_steps_done": done,
-                     "still_active": active}
+        per_step = timed(b, E2D_BATCH_LAUNCH_STEPS)
+        out[name] = timed(b, E2D_BATCH_LAUNCH_PERSISTENT)  # the default, run last: its
+        out[name]["per_step_launches"] = per_step         # state is the one compared
         runs[name] = b
     err = density_l1(runs["fp32_split"], runs["fp64"])
     worst = int(np.argmax(err))
@@ -163,7 +176,8 @@ def batch_line(steps: int) -> dict:
             "instances": info["count"], "cells": info["cells"], "steps_each": steps,
             "instance_steps_done": split["instance_steps_done"],
             "still_active": split["still_active"], "tile": split["tile"],
-            "device_ms": split["device_ms"], "precision": "fp32_split", "launch": "hipGraph",
+            "device_ms": split["device_ms"], "precision": "fp32_split",
+            "launch": "persistent", "per_step_launches": split["per_step_launches"],
             "fp64": out["fp64"],
             "speedup_vs_fp64": round(split["value"] / out["fp64"]["value"], 3),
             "rel_l1_density_vs_fp64": {

@@ -25,6 +25,8 @@ E2D_ERR_UNSUPPORTED = -5
 E2D_U = 0
 E2D_U2 = 1
 
+E2D_BATCH_LAUNCH_STEPS = 0
+E2D_BATCH_LAUNCH_PERSISTENT = 1
 E2D_LAYOUT_RIGHT = 0
 E2D_LAYOUT_LEFT = 1
 E2D_LAYOUT_INTERIOR = 2
@@ -141,6 +143,8 @@ _PROTOTYPES = {
     "e2d_batch_destroy": (C.c_int, [C.c_void_p]),
     "e2d_batch_init": (C.c_int, [C.c_void_p]),
     "e2d_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
+    "e2d_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "e2d_batch_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "e2d_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
                                 C.POINTER(C.c_int)]),
     "e2d_batch_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),

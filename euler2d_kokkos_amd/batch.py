@@ -86,6 +86,18 @@ class HydroBatch:
     def set_tile(self, seg_rows: int) -> None:
         check(load_library().e2d_batch_set_tile(self._h, seg_rows))
 
+    def set_launch(self, mode: int, spin_limit: int = 0) -> None:
+        """E2D_BATCH_LAUNCH_PERSISTENT (all the steps of a run in one launch,
+        the default) or E2D_BATCH_LAUNCH_STEPS (one launch per step, replayed
+        from hipGraphs); spin_limit bounds each dependency wait of the
+        persistent launch."""
+        check(load_library().e2d_batch_set_launch(self._h, mode, spin_limit))
+
+    def get_launch(self) -> int:
+        out = C.c_int(-1)
+        check(load_library().e2d_batch_get_launch(self._h, C.byref(out)))
+        return out.value
+
     def run(self, steps: int, use_graph: bool = True) -> tuple[float, int]:
         """Runs `steps` batched steps; returns (device ms, instances still active)."""
         ms = C.c_double(0.0)

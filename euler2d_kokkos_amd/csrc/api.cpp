@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1095,6 +1096,16 @@ struct e2d_batch {
   int graph_steps = 0;
   int graph_tile = 0;
   int floor_sqrt = 1;  // every instance allows the short wave-speed square root
+  // persistent launch (E2D_BATCH_LAUNCH_PERSISTENT)
+  int launch = E2D_BATCH_LAUNCH_PERSISTENT;
+  unsigned spin_limit = 0;
+  int nxcd = 0;                          // XCDs seen by the probe at creation
+  int xmap[e2d::kMaxXcds];               // XCC id -> queue (-1: not seen)
+  int ncu = 0;                           // CUs of the device
+  int *queue_ints = nullptr;             // device: order, wave_prefix, xcd_begin, per_step
+  e2d::BatchQueue queue{};
+  std::vector<int> queue_per_step;       // host copy of queue.per_step
+  e2d::BatchCounters *counters = nullptr;  // device, followed by done[count * kCounterStride]
 };
 
 namespace {
@@ -1116,11 +1127,62 @@ void batch_drop_graph(e2d_batch *b) {
   b->graph_steps = 0;
 }
 
+// default bound of a persistent dependency wait: sleep rounds of about 64
+// cycles, about a second in all
+constexpr unsigned kDefaultSpinLimit = 1u << 25;
+
+// Work queues of the persistent launch for the current tile: instances go to
+// the XCD with the least work so far, largest first (cells as the measure);
+// within an XCD the largest instances come first in each step.
+int batch_upload_queue(e2d_batch *b, const std::vector<int> &waves) {
+  const int n = (int)b->runs.size(), nx = b->nxcd;
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  auto cells = [&](int i) { return (long long)b->runs[i]->p.nx * b->runs[i]->p.ny; };
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return cells(a) > cells(c); });
+  std::vector<long long> load(nx, 0);
+  std::vector<std::vector<int>> group(nx);
+  for (int i : idx) {
+    int x = 0;
+    for (int k = 1; k < nx; ++k)
+      if (load[k] < load[x]) x = k;
+    load[x] += cells(i);
+    group[x].push_back(i);
+  }
+  std::vector<int> order, prefix, begin(nx + 1, 0), per(nx, 0);
+  for (int x = 0; x < nx; ++x) {
+    begin[x] = (int)order.size();
+    for (int i : group[x]) {
+      order.push_back(i);
+      prefix.push_back(per[x]);
+      per[x] += waves[i];
+    }
+  }
+  begin[nx] = (int)order.size();
+  b->queue_per_step = per;
+  std::vector<int> h;
+  h.insert(h.end(), order.begin(), order.end());
+  h.insert(h.end(), prefix.begin(), prefix.end());
+  h.insert(h.end(), begin.begin(), begin.end());
+  h.insert(h.end(), per.begin(), per.end());
+  HIP_TRY(hipMemcpyAsync(b->queue_ints, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice,
+                         b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));  // h is read before it goes
+  b->queue.order = b->queue_ints;
+  b->queue.wave_prefix = b->queue_ints + n;
+  b->queue.xcd_begin = b->queue_ints + 2 * n;
+  b->queue.per_step = b->queue_ints + 2 * n + nx + 1;
+  b->queue.nxcd = nx;
+  for (int k = 0; k < e2d::kMaxXcds; ++k) b->queue.xmap[k] = b->xmap[k];
+  return E2D_OK;
+}
+
 // instance table for the current tile
 int batch_upload_table(e2d_batch *b) {
   std::vector<e2d::BatchInst> h(b->runs.size());
   int waves = 0;
   long long cells = 0;
+  std::vector<int> inst_waves(b->runs.size());
   for (size_t i = 0; i < b->runs.size(); ++i) {
     e2d_run *r = b->runs[i];
     e2d::BatchInst &bi = h[i];
@@ -1129,13 +1191,16 @@ int batch_upload_table(e2d_batch *b) {
     bi.U = r->U;
     bi.U2 = r->U2;
     bi.wave_begin = waves;
-    waves += e2d::batch_waves(r->kp, b->seg_rows, &bi.nstrips);
+    inst_waves[i] = e2d::batch_waves(r->kp, b->seg_rows, &bi.nstrips);
+    waves += inst_waves[i];
     cells += (long long)r->p.nx * r->p.ny;
   }
   b->nwaves = waves;
   b->cells = cells;
   HIP_TRY(hipMemcpyAsync(b->insts, h.data(), h.size() * sizeof(e2d::BatchInst),
                          hipMemcpyHostToDevice, b->stream));
+  int rc = batch_upload_queue(b, inst_waves);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   batch_drop_graph(b);
   return E2D_OK;
@@ -1199,9 +1264,85 @@ int batch_active(e2d_batch *b, int *active) {
   return E2D_OK;
 }
 
+// All max_steps steps in one persistent launch (E2D_BATCH_LAUNCH_PERSISTENT),
+// or in as few as keep the launch's item count (steps x the largest XCD queue
+// step) below 2^31.
+int batch_run_persistent(e2d_batch *b, int max_steps, double *elapsed_ms, int *active_left) {
+  e2d::KParams k0 = b->runs[0]->kp;
+  k0.s.floor_sqrt = b->floor_sqrt;
+  for (const e2d_run *r : b->runs) k0.wide_addr |= r->kp.wide_addr;
+  const size_t cbytes =
+      sizeof(e2d::BatchCounters) + sizeof(unsigned) * e2d::kCounterStride * b->runs.size();
+  unsigned *done = reinterpret_cast<unsigned *>(b->counters + 1);
+  long long per_max = 1;
+  for (int x = 0; x < b->nxcd; ++x) per_max = std::max(per_max, (long long)b->queue_per_step[x]);
+  const int cap = (int)std::min<long long>(INT_MAX, ((1LL << 31) - 1) / per_max);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, b->stream));
+  unsigned err = 0;
+  for (int left = max_steps; left > 0 && !err;) {
+    const int steps = std::min(left, cap);
+    HIP_TRY(hipMemsetAsync(b->counters, 0, cbytes, b->stream));
+    HIP_TRY(e2d::launch_step_batch_persistent(
+        k0, b->insts, (int)b->runs.size(), b->ctls, b->phase, steps, b->queue, b->counters,
+        done, b->seg_rows, b->spin_limit ? b->spin_limit : kDefaultSpinLimit, b->ncu,
+        b->stream));
+    b->phase = (b->phase + steps) % 3;
+    left -= steps;
+    e2d::BatchCounters c{};
+    HIP_TRY(hipMemcpyAsync(&c, b->counters, sizeof(c), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    err = c.err;
+    // every queue was drained (an XCD that ran no wavefront would leave its
+    // instances behind)
+    for (int x = 0; x < b->nxcd && !err; ++x)
+      if (c.head[x][0] < (unsigned)steps * (unsigned)b->queue_per_step[x]) err = 2;
+  }
+  HIP_TRY(hipEventRecord(e1, b->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err == 2) {
+    b->ready = false;
+    return fail(E2D_ERR_HIP, "persistent batch launch: an XCD queue was not drained "
+                             "(the instances are part-way; run e2d_batch_init)");
+  }
+  if (err) {
+    b->ready = false;
+    return fail(E2D_ERR_HIP, "persistent batch launch: a dependency wait ran past its limit "
+                             "(the instances are part-way; run e2d_batch_init)");
+  }
+  if (elapsed_ms) *elapsed_ms = ms;
+  int act = 0;
+  int rc = batch_active(b, &act);
+  if (rc) return rc;
+  if (active_left) *active_left = act;
+  return E2D_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int e2d_batch_set_launch(e2d_batch *b, int mode, int spin_limit) {
+  int rc = batch_check(b);
+  if (rc) return rc;
+  if (mode != E2D_BATCH_LAUNCH_STEPS && mode != E2D_BATCH_LAUNCH_PERSISTENT)
+    return fail(E2D_ERR_ARG, "mode must be E2D_BATCH_LAUNCH_STEPS or _PERSISTENT");
+  b->launch = mode;
+  b->spin_limit = spin_limit > 0 ? (unsigned)spin_limit : 0u;
+  return E2D_OK;
+}
+
+int e2d_batch_get_launch(e2d_batch *b, int *mode) {
+  if (!b || !mode) return fail(E2D_ERR_ARG, "null argument");
+  *mode = b->launch;
+  return E2D_OK;
+}
 
 int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_batch **out) {
   if (!params || !out || count <= 0) return fail(E2D_ERR_ARG, "bad argument");
@@ -1239,6 +1380,41 @@ int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_bat
     }
     b->insts = static_cast<e2d::BatchInst *>(p1);
     b->ctls = static_cast<e2d::StepCtl *>(p2);
+    // persistent launch: queue tables, counters, and the XCDs the device has
+    // (the highest XCC id seen by 4096 one-wavefront blocks, plus one)
+    void *p3 = nullptr, *p4 = nullptr, *p5 = nullptr;
+    const size_t qints = 2 * (size_t)count + 2 * e2d::kMaxXcds + 1;
+    if (hipMalloc(&p3, qints * sizeof(int)) != hipSuccess ||
+        hipMalloc(&p4, sizeof(e2d::BatchCounters) +
+                           sizeof(unsigned) * e2d::kCounterStride * count) != hipSuccess ||
+        hipMalloc(&p5, 4096 * sizeof(unsigned)) != hipSuccess) {
+      for (void *q : {p3, p4, p5})
+        if (q) (void)hipFree(q);
+      e2d_batch_destroy(b);
+      return fail(E2D_ERR_ALLOC, "batch queue tables");
+    }
+    b->queue_ints = static_cast<int *>(p3);
+    b->counters = static_cast<e2d::BatchCounters *>(p4);
+    std::vector<unsigned> xcc(4096, 0);
+    e = e2d::launch_xcc_probe(static_cast<unsigned *>(p5), 4096, b->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(xcc.data(), p5, xcc.size() * sizeof(unsigned), hipMemcpyDeviceToHost,
+                         b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(p5);
+    if (e != hipSuccess) {
+      e2d_batch_destroy(b);
+      return hip_fail(e, "XCC probe");
+    }
+    // one queue per XCC id that blocks ran on
+    for (int k = 0; k < e2d::kMaxXcds; ++k) b->xmap[k] = -1;
+    for (unsigned x : xcc) b->xmap[x & (e2d::kMaxXcds - 1)] = 0;
+    b->nxcd = 0;
+    for (int k = 0; k < e2d::kMaxXcds; ++k)
+      if (b->xmap[k] == 0) b->xmap[k] = b->nxcd++;
+    int ncu = 0;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device);
+    b->ncu = std::max(ncu, 1);
     e = hipMemsetAsync(b->ctls, 0, sizeof(e2d::StepCtl) * count, b->stream);
     if (e != hipSuccess) {
       e2d_batch_destroy(b);
@@ -1263,6 +1439,8 @@ int e2d_batch_destroy(e2d_batch *b) {
   for (e2d_run *r : b->runs) e2d_destroy(r);
   if (b->insts) (void)hipFree(b->insts);
   if (b->ctls) (void)hipFree(b->ctls);
+  if (b->queue_ints) (void)hipFree(b->queue_ints);
+  if (b->counters) (void)hipFree(b->counters);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return E2D_OK;
@@ -1303,6 +1481,8 @@ int e2d_batch_run(e2d_batch *b, int max_steps, int use_graph, double *elapsed_ms
   if (rc) return rc;
   if (max_steps <= 0) return fail(E2D_ERR_ARG, "max_steps must be > 0");
   if (!b->ready) return fail(E2D_ERR_ARG, "e2d_batch_init has not run on this batch");
+  if (b->launch == E2D_BATCH_LAUNCH_PERSISTENT) return batch_run_persistent(b, max_steps,
+                                                                            elapsed_ms, active_left);
   if (use_graph) {
     rc = batch_graph(b);
     if (rc) return rc;
@@ -1381,10 +1561,17 @@ int e2d_batch_autotune(e2d_batch *b, const int *candidates, int ncand, int steps
       if (rc) return rc;
       double ms = 0.0;
       int act = 0;
-      rc = e2d_batch_run(b, 2, 0, &ms, &act);  // warm-up
-      if (rc) return rc;
-      rc = e2d_batch_run(b, steps, 0, &ms, &act);
-      if (rc) return rc;
+      for (int attempt = 0; attempt < 2; ++attempt) {
+        rc = e2d_batch_run(b, 2, 0, &ms, &act);  // warm-up
+        if (rc) return rc;
+        rc = e2d_batch_run(b, steps, 0, &ms, &act);
+        if (rc) return rc;
+        // a run in which instances reached the end of their loops timed
+        // launches that returned at once: start the batch again and repeat it
+        if (act == (int)b->runs.size() || attempt == 1) break;
+        rc = e2d_batch_init(b);
+        if (rc) return rc;
+      }
       cand_ms[(size_t)c] = std::min(cand_ms[(size_t)c], ms / steps);
     }
   }

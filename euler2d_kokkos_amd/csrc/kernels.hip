@@ -1288,9 +1288,9 @@ __global__ __launch_bounds__(kBlock) void step_fused_kernel(KParams kp, Grid in,
 // in/out arrays follow the instance's own step parity.  Each instance has its
 // own loop control rings (StepCtl), published by its first wavefront.
 template <int RS, class P, bool O32>
-__global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__restrict__ insts,
-                                                            int ninst, StepCtl *ctls, int ph,
-                                                            int nwaves, int seg_rows) {
+__device__ __forceinline__ void batch_step_wave(const BatchInst *__restrict__ insts, int ninst,
+                                                StepCtl *ctls, int ph, int nwaves,
+                                                int seg_rows) {
   using T = typename P::T;
   const int wave =
       __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kLanes) + (threadIdx.x >> 6));
@@ -1331,6 +1331,215 @@ __global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__r
     }
   }
   reduce_rate_wave(wmax, target);
+}
+
+template <int RS, class P, bool O32>
+__global__ __launch_bounds__(kBlock) void step_batch_kernel(const BatchInst *__restrict__ insts,
+                                                            int ninst, StepCtl *ctls, int ph,
+                                                            int nwaves, int seg_rows) {
+  batch_step_wave<RS, P, O32>(insts, ninst, ctls, ph, nwaves, seg_rows);
+}
+
+// The split-precision step held to three wavefronts per SIMD (168 VGPRs, a few
+// words spilled), where the unconstrained build (163-169 VGPRs with the
+// 32-bit row offsets) may fit only two: 49.3 against 45.4 Gcell/s on the
+// 512-instance sweep.
+template <int RS, class P, bool O32>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void step_batch_kernel_w3(const BatchInst *__restrict__ insts, int ninst, StepCtl *ctls, int ph,
+                          int nwaves, int seg_rows) {
+  batch_step_wave<RS, P, O32>(insts, ninst, ctls, ph, nwaves, seg_rows);
+}
+
+// ----------------------------------------------- persistent batched step kernel
+//
+// All the steps of a batch in one launch.  A wavefront reads the id of the XCD
+// it runs on and takes work items from that XCD's queue (BatchQueue): step n
+// of instance i, strip / segment w.  Before marching it waits until every
+// item of step n - 1 of instance i has completed (per-instance count in
+// `done`), so an instance advances as soon as its own previous step is done,
+// with no grid-wide barrier between steps.  All items of an instance run on
+// one XCD, whose L2 holds the rows its items write: a completed item's stores
+// have reached that L2 (s_waitcnt before the count), and the next item drops
+// its CU's L1 lines before reading.  The control rings are read and written
+// with atomics, at the point where the rate atomics are performed.  Items are
+// taken in step-major order, so an item only waits for items already taken
+// by running wavefronts; every wait is bounded all the same (spin_limit sleep
+// rounds), and a wait that runs out sets counters->err and ends the launch.
+
+__device__ __forceinline__ unsigned xcc_id() {
+  unsigned x;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+  return x & (kMaxXcds - 1);
+}
+
+__global__ void xcc_probe_kernel(unsigned *out) {
+  const unsigned x = xcc_id();
+  if (threadIdx.x == 0) out[blockIdx.x] = x;
+}
+
+// A word as the atomics left it: a device-scope atomic load, which the
+// compiler issues as a vector load that bypasses the CU's caches (never a
+// scalar load through the scalar cache, which atomics do not update).
+__device__ __forceinline__ unsigned long long ctl_read(unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned counter_read(unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ LoopPos loop_pos_coherent(const KParams &kp, StepCtl *ctl, int ph) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  unsigned long long w0 = 0, w1 = 0;
+  if (lane == 0) {
+    unsigned long long *s = reinterpret_cast<unsigned long long *>(&ctl->s[ph]);
+    w0 = ctl_read(s);
+    w1 = ctl_read(s + 1);
+  }
+  const unsigned long long b =
+      lane < kRateSlots ? ctl_read(&ctl->rate[ph][lane * kSlotStride]) : 0ull;
+  LoopPos L;
+  // readfirstlane returns an int: the halves are taken as unsigned, so that a
+  // low half with its top bit set is not sign-extended into the high half
+  const unsigned tlo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)w0);
+  const unsigned thi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(w0 >> 32));
+  L.t = __longlong_as_double((long long)(((unsigned long long)thi << 32) | tlo));
+  L.nstep = (int)__builtin_amdgcn_readfirstlane((unsigned)w1);
+  L.active = L.t < kp.tEnd && L.nstep < kp.nStepmax;
+  const long long mb = __double_as_longlong(wave_max(__longlong_as_double((long long)b)));
+  const unsigned mlo = __builtin_amdgcn_readfirstlane((unsigned)mb);
+  const unsigned mhi = __builtin_amdgcn_readfirstlane((unsigned)(mb >> 32));
+  const double invdt = __longlong_as_double((long long)(((unsigned long long)mhi << 32) | mlo));
+  double dt = kp.cfl / invdt;
+  if (L.t + dt > kp.tEnd) dt = kp.tEnd - L.t;
+  L.dt = dt;
+  return L;
+}
+
+__device__ __forceinline__ void loop_publish_coherent(StepCtl *ctl, int ph, const LoopPos &L,
+                                                      int lane) {
+  const int p1 = ph == 2 ? 0 : ph + 1;
+  const int p2 = ph == 0 ? 2 : ph - 1;
+  if (lane < kRateSlots) atomicExch(&ctl->rate[p2][lane * kSlotStride], 0ull);
+  if (lane == 0) {
+    const double t = L.active ? L.t + L.dt : L.t;
+    const int n = L.active ? L.nstep + 1 : L.nstep;
+    unsigned long long *s = reinterpret_cast<unsigned long long *>(&ctl->s[p1]);
+    atomicExch(s, (unsigned long long)__double_as_longlong(t));
+    atomicExch(s + 1, (unsigned long long)(unsigned)n);
+  }
+}
+
+// The loop of one persistent wavefront.  Items are taken one at a time in
+// queue order and marched at once, so the lowest item not yet completed is
+// always held by a running wavefront whose waits are all on lower items.
+template <int RS, class P, bool O32>
+__device__ __forceinline__ void batch_persistent_loop(const BatchInst *__restrict__ insts,
+                                                      StepCtl *ctls, int ph0, int steps,
+                                                      const BatchQueue &q, BatchCounters *cnt,
+                                                      unsigned *done, int seg_rows,
+                                                      unsigned spin_limit) {
+  using T = typename P::T;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int xq = q.xmap[xcc_id()];
+  if (xq < 0) return;
+  const unsigned per = (unsigned)q.per_step[xq];
+  if (per == 0) return;
+  const unsigned total = (unsigned)steps * per;
+  const int b0 = q.xcd_begin[xq], b1 = q.xcd_begin[xq + 1];
+  for (;;) {
+    unsigned h = 0;
+    if (lane == 0) h = atomicAdd(&cnt->head[xq][0], 1u);
+    const unsigned it = __builtin_amdgcn_readfirstlane(h);
+    if (it >= total) break;
+    const int n = (int)(it / per);
+    const int r = (int)(it - (unsigned)n * per);
+    int lo = b0, hi = b1 - 1;
+    while (lo < hi) {  // last position whose first item is at or before r
+      const int mid = (lo + hi + 1) >> 1;
+      if (q.wave_prefix[mid] <= r)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int pos = __builtin_amdgcn_readfirstlane(lo);
+    const int inst = q.order[pos];
+    const int w = r - q.wave_prefix[pos];
+    const BatchInst &bi = insts[inst];
+    const KParams kp = bi.kp;
+    if (n > 0) {  // every item of step n - 1 of this instance has completed
+      const unsigned target = (unsigned)n * (unsigned)bi.nstrips *
+                              (unsigned)((kp.ny + seg_rows - 1) / seg_rows);
+      bool ok = true;
+      for (unsigned spins = 0;; ++spins) {
+        unsigned d = 0, e = 0;
+        if (lane == 0) {
+          d = counter_read(&done[inst * kCounterStride]);
+          e = counter_read(&cnt->err);
+        }
+        d = __builtin_amdgcn_readfirstlane(d);
+        e = __builtin_amdgcn_readfirstlane(e);
+        if (d >= target) break;
+        if (e != 0u || spins >= spin_limit) {
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (!ok) {
+        if (lane == 0) atomicOr(&cnt->err, 1u);
+        break;
+      }
+    }
+    // rows of instance i that this CU read in an earlier step may still sit in
+    // its L1: drop them (the L2 of this XCD has the new ones).  The invalidate
+    // takes agent scope (sc1): at workgroup scope (sc0) the CU's L1 counts as
+    // coherent for its own workgroup and is left as it is.
+    asm volatile("buffer_inv sc1" ::: "memory");
+    const int ph = (ph0 + n) % 3;
+    StepCtl *ctl = ctls + inst;
+    const LoopPos L = loop_pos_coherent(kp, ctl, ph);
+    if (w == 0) loop_publish_coherent(ctl, ph, L, lane);
+    T wmax = T(0.0);
+    if (L.active) {
+      const bool even = (L.nstep & 1) == 0;
+      const Grid gin = even ? bi.U : bi.U2;
+      Grid gout = gin;
+      gout.p = even ? bi.U2.p : bi.U.p;
+      const int strip = w % bi.nstrips, seg = w / bi.nstrips;
+      if (interior_cols(kp, strip))
+        wmax = march<RS, P, true, O32, false, 1>(kp, gin, gout, L.dt, seg_rows, bi.nstrips,
+                                                 strip, seg, false);
+      else
+        wmax = march<RS, P, true, O32, true, 1>(kp, gin, gout, L.dt, seg_rows, bi.nstrips,
+                                                strip, seg, false);
+      reduce_rate_wave(wmax, rate_slot(ctl, ph));
+    }
+    // this wavefront's stores and atomics are complete before it counts
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (lane == 0) atomicAdd(&done[inst * kCounterStride], 1u);
+  }
+}
+
+template <int RS, class P, bool O32>
+__global__ __launch_bounds__(kBlock) void step_batch_persistent_kernel(
+    const BatchInst *__restrict__ insts, StepCtl *ctls, int ph0, int steps, BatchQueue q,
+    BatchCounters *cnt, unsigned *done, int seg_rows, unsigned spin_limit) {
+  batch_persistent_loop<RS, P, O32>(insts, ctls, ph0, steps, q, cnt, done, seg_rows, spin_limit);
+}
+
+// The split-precision loop held to three wavefronts per SIMD (168 VGPRs; the
+// compiler spills a few words): a wavefront that waits on a queue or a
+// dependency leaves two to issue, where the unconstrained build (175-180
+// VGPRs) fits two per SIMD.  The FP64 loop would spill 13-23 words and stays
+// at two.
+template <int RS, class P, bool O32>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void step_batch_persistent_kernel_w3(const BatchInst *__restrict__ insts, StepCtl *ctls, int ph0,
+                                     int steps, BatchQueue q, BatchCounters *cnt,
+                                     unsigned *done, int seg_rows, unsigned spin_limit) {
+  batch_persistent_loop<RS, P, O32>(insts, ctls, ph0, steps, q, cnt, done, seg_rows, spin_limit);
 }
 
 // ----------------------------------------------------------- radial profile
@@ -1710,34 +1919,89 @@ int batch_waves(const KParams &kp, int seg_rows, int *nstrips) {
   return ns * ((kp.ny + seg_rows - 1) / seg_rows);
 }
 
+hipError_t launch_xcc_probe(unsigned *out, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(xcc_probe_kernel, dim3((unsigned)blocks), dim3(kLanes), 0, st, out);
+  return hipGetLastError();
+}
+
+// One block of four wavefronts per slot the kernel's registers leave on a CU,
+// times the CUs: every block is resident from the start, and none waits for
+// another to exit.
+template <class K>
+hipError_t launch_persistent(K kernel, int ncu, hipStream_t st, const BatchInst *insts_dev,
+                             StepCtl *ctls_dev, int ph0, int steps, const BatchQueue &q,
+                             BatchCounters *counters, unsigned *done, int seg_rows,
+                             unsigned spin_limit) {
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  hipLaunchKernelGGL(kernel, dim3(per_cu * ncu), dim3(kBlock), 0, st, insts_dev, ctls_dev, ph0,
+                     steps, q, counters, done, seg_rows, spin_limit);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_batch_persistent(const KParams &kp0, const BatchInst *insts_dev, int ninst,
+                                        StepCtl *ctls_dev, int ph0, int steps, BatchQueue q,
+                                        BatchCounters *counters, unsigned *done, int seg_rows,
+                                        unsigned spin_limit, int ncu, hipStream_t st) {
+  (void)ninst;
+  if (ph0 < 0 || ph0 > 2 || steps <= 0 || ncu <= 0) return hipErrorInvalidValue;
+#define E2D_PBATCH_ARGS ncu, st, insts_dev, ctls_dev, ph0, steps, q, counters, done, seg_rows, \
+                        spin_limit
+#define E2D_PBATCH_LAUNCH(K, RS, P)                                            \
+  return kp0.wide_addr ? launch_persistent(K<RS, P, false>, E2D_PBATCH_ARGS) \
+                       : launch_persistent(K<RS, P, true>, E2D_PBATCH_ARGS)
+  if (kp0.prec == E2D_PRECISION_FP32_SPLIT) {
+    if (kp0.s.riemann == 2) E2D_PBATCH_LAUNCH(step_batch_persistent_kernel_w3, 2, PSplit);
+    if (kp0.s.riemann == 1) E2D_PBATCH_LAUNCH(step_batch_persistent_kernel_w3, 1, PSplit);
+    E2D_PBATCH_LAUNCH(step_batch_persistent_kernel_w3, 0, PSplit);
+  }
+  if (kp0.prec == E2D_PRECISION_FP32) {
+    if (kp0.s.riemann == 2) E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 2, P32);
+    if (kp0.s.riemann == 1) E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 1, P32);
+    E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 0, P32);
+  }
+  if (kp0.s.floor_sqrt && kp0.s.riemann != 0) {
+    if (kp0.s.riemann == 2)
+      E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 2 | kRiemannFloorSqrt, P64);
+    E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 1 | kRiemannFloorSqrt, P64);
+  }
+  if (kp0.s.riemann == 2) E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 2, P64);
+  if (kp0.s.riemann == 1) E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 1, P64);
+  E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 0, P64);
+#undef E2D_PBATCH_LAUNCH
+#undef E2D_PBATCH_ARGS
+}
+
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
                              StepCtl *ctls_dev, int ph, int nwaves, int seg_rows, hipStream_t st) {
   if (ph < 0 || ph > 2) return hipErrorInvalidValue;
   const int blocks = (nwaves + kBlock / kLanes - 1) / (kBlock / kLanes);
-#define E2D_BATCH_LAUNCH(RS, P)                                                            \
-  do {                                                                                     \
-    if (kp0.wide_addr)                                                                     \
-      hipLaunchKernelGGL((step_batch_kernel<RS, P, false>), dim3(blocks), dim3(kBlock), 0,   \
-                         st, insts_dev, ninst, ctls_dev, ph, nwaves, seg_rows);               \
-    else                                                                                   \
-      hipLaunchKernelGGL((step_batch_kernel<RS, P, true>), dim3(blocks), dim3(kBlock), 0,    \
-                         st, insts_dev, ninst, ctls_dev, ph, nwaves, seg_rows);               \
+#define E2D_BATCH_LAUNCH(K, RS, P)                                                            \
+  do {                                                                                        \
+    if (kp0.wide_addr)                                                                        \
+      hipLaunchKernelGGL((K<RS, P, false>), dim3(blocks), dim3(kBlock), 0, st, insts_dev,     \
+                         ninst, ctls_dev, ph, nwaves, seg_rows);                               \
+    else                                                                                      \
+      hipLaunchKernelGGL((K<RS, P, true>), dim3(blocks), dim3(kBlock), 0, st, insts_dev,      \
+                         ninst, ctls_dev, ph, nwaves, seg_rows);                               \
   } while (0)
   if (kp0.prec == E2D_PRECISION_FP32_SPLIT) {
-    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, PSplit);
-    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, PSplit);
-    else E2D_BATCH_LAUNCH(0, PSplit);
+    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(step_batch_kernel_w3, 2, PSplit);
+    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(step_batch_kernel_w3, 1, PSplit);
+    else E2D_BATCH_LAUNCH(step_batch_kernel_w3, 0, PSplit);
   } else if (kp0.prec == E2D_PRECISION_FP32) {
-    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, P32);
-    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, P32);
-    else E2D_BATCH_LAUNCH(0, P32);
+    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(step_batch_kernel, 2, P32);
+    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(step_batch_kernel, 1, P32);
+    else E2D_BATCH_LAUNCH(step_batch_kernel, 0, P32);
   } else if (kp0.s.floor_sqrt && kp0.s.riemann != 0) {
-    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2 | kRiemannFloorSqrt, P64);
-    else E2D_BATCH_LAUNCH(1 | kRiemannFloorSqrt, P64);
+    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(step_batch_kernel, 2 | kRiemannFloorSqrt, P64);
+    else E2D_BATCH_LAUNCH(step_batch_kernel, 1 | kRiemannFloorSqrt, P64);
   } else {
-    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(2, P64);
-    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(1, P64);
-    else E2D_BATCH_LAUNCH(0, P64);
+    if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(step_batch_kernel, 2, P64);
+    else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(step_batch_kernel, 1, P64);
+    else E2D_BATCH_LAUNCH(step_batch_kernel, 0, P64);
   }
 #undef E2D_BATCH_LAUNCH
   return hipGetLastError();

@@ -84,6 +84,33 @@ struct BatchInst {
   int pad[2];
 };
 
+// Work queues of the persistent batched step.  Every instance is assigned to
+// one XCD (accelerator complex die, with its own L2); the wavefronts running on
+// that XCD take the instance's work items, so the rows one step writes are
+// read by the next from the same L2.  Items of a queue are ordered step-major:
+// step n of every instance of the XCD, then step n + 1.  An item is one
+// wavefront's march (strip, segment) of one instance.
+constexpr int kMaxXcds = 16;  // the XCC id is a 4-bit field
+struct BatchQueue {
+  const int *order;        // instance ids grouped by XCD (xcd_begin[x] .. xcd_begin[x + 1])
+  const int *wave_prefix;  // per position of `order`: wavefronts of the XCD's earlier instances
+  const int *xcd_begin;    // nxcd + 1 offsets into order
+  const int *per_step;     // per XCD: items (wavefronts) of one step of all its instances
+  int nxcd;
+  int xmap[kMaxXcds];      // XCC id -> queue (-1: no queue; the probe saw no block there)
+};
+// Counters of one persistent run, zeroed before the launch: the next item of
+// each XCD's queue, an error flag (a dependency wait ran past its limit), and
+// then, per instance, the number of its items completed (done[i * kCounterStride]).
+// Every counter has a 256-byte line of its own: device-scope atomics on one
+// line serialise, and each queue head takes one atomic per item taken.
+constexpr int kCounterStride = 64;
+struct BatchCounters {
+  unsigned head[kMaxXcds][kCounterStride];
+  unsigned err;
+  unsigned pad[kCounterStride - 1];
+};
+
 // Geometry of the radial profile (ComputeRadialProfileFunctor.h:83-100)
 struct RadialGeom {
   double xmin, ymin, cx, cy, max_radial_distance;
@@ -162,6 +189,18 @@ int batch_waves(const KParams &kp, int seg_rows, int *nstrips);
 // when any instance has a plane above 4 GiB)
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
                              StepCtl *ctls_dev, int ph, int nwaves, int seg_rows, hipStream_t st);
+// All `steps` steps of every instance in one persistent launch: as many blocks
+// of 256 threads as the kernel fits on `ncu` CUs at once.  counters and done
+// (one word every kCounterStride per instance) must be zero; steps times the
+// largest per_step must stay below 2^31.
+// spin_limit bounds every dependency wait (sleep rounds); a wait that runs
+// past it sets counters->err and the launch ends early.
+hipError_t launch_step_batch_persistent(const KParams &kp0, const BatchInst *insts_dev, int ninst,
+                                        StepCtl *ctls_dev, int ph0, int steps, BatchQueue q,
+                                        BatchCounters *counters, unsigned *done, int seg_rows,
+                                        unsigned spin_limit, int ncu, hipStream_t st);
+// XCC id of the XCD each of `blocks` one-wavefront blocks ran on, into out[block]
+hipError_t launch_xcc_probe(unsigned *out, int blocks, hipStream_t st);
 // per-bin cell counts and density sums of the radial profile (nbins entries each)
 hipError_t launch_radial_profile(const KParams &kp, Grid U, const RadialGeom &geo,
                                  unsigned long long *counts, double *sums, hipStream_t st);

@@ -298,6 +298,24 @@ int e2d_batch_destroy(e2d_batch *batch);
  * instance (main.cpp:87-91); resets t and nStep to 0 */
 int e2d_batch_init(e2d_batch *batch);
 int e2d_batch_set_tile(e2d_batch *batch, int seg_rows);
+/* How e2d_batch_run launches the steps (no reference counterpart: the
+ * reference runs one grid per process, src/main.cpp:86).
+ *   E2D_BATCH_LAUNCH_PERSISTENT  (the default) all the run's steps in one
+ *                                launch: every instance is assigned to one
+ *                                XCD, whose wavefronts take its work items
+ *                                from that XCD's queue and start its step
+ *                                n + 1 as soon as its own step n has
+ *                                completed; use_graph is ignored
+ *   E2D_BATCH_LAUNCH_STEPS       one launch per step for all instances,
+ *                                replayed from hipGraphs when use_graph != 0
+ * spin_limit bounds each dependency wait of the persistent launch (sleep
+ * rounds; <= 0: the default, about a second); a wait that runs past it ends
+ * the launch and e2d_batch_run returns E2D_ERR_HIP, leaving the instances
+ * part-way (run e2d_batch_init).  Results are the same bits either way. */
+#define E2D_BATCH_LAUNCH_STEPS 0
+#define E2D_BATCH_LAUNCH_PERSISTENT 1
+int e2d_batch_set_launch(e2d_batch *batch, int mode, int spin_limit);
+int e2d_batch_get_launch(e2d_batch *batch, int *mode);
 /* Runs up to max_steps batched steps (instances whose loop has ended stay put).
  * use_graph != 0 replays captured hipGraphs of the step launch.  Returns the
  * device time in milliseconds and the number of instances still active.
@@ -309,9 +327,11 @@ int e2d_batch_status(e2d_batch *batch, int instance, double *t, int *nStep);
 int e2d_batch_copy_state(e2d_batch *batch, int instance, double *host, int layout);
 /* total interior cells, wavefronts per step, instance count */
 int e2d_batch_info(e2d_batch *batch, long long *cells, int *waves, int *count);
-/* Times each candidate seg_rows over `steps` batched steps, in two
- * interleaved passes (restoring the state afterwards is the caller's business:
- * call e2d_batch_init), and keeps the fastest; returns it. */
+/* Times each candidate seg_rows over `steps` batched steps in the batch's
+ * launch mode, in two interleaved passes, and keeps the fastest; returns it.  A candidate whose run
+ * saw instances reach the end of their loops is timed again after
+ * e2d_batch_init.  Restoring the state afterwards is the caller's business:
+ * call e2d_batch_init. */
 int e2d_batch_autotune(e2d_batch *batch, const int *candidates, int ncand, int steps,
                        int *best_seg_rows, double *best_ms_per_step);
 

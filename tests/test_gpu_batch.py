@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from euler2d_kokkos_amd import (E2D_PRECISION_FP32, E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64,
+from euler2d_kokkos_amd import (E2D_BATCH_LAUNCH_PERSISTENT, E2D_BATCH_LAUNCH_STEPS,
+                                E2D_PRECISION_FP32, E2D_PRECISION_FP32_SPLIT, E2D_PRECISION_FP64,
                                 HydroParams, HydroRun)
 from euler2d_kokkos_amd.batch import HydroBatch, density_l1, make_params, sweep_params
 
@@ -32,12 +33,15 @@ def mixed_instances():
 
 @pytest.mark.parametrize("precision", [E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT,
                                        E2D_PRECISION_FP32], ids=["fp64", "split", "fp32"])
-@pytest.mark.parametrize("use_graph", [False, True], ids=["launches", "graph"])
-def test_batch_equals_single_runs(gpu_available, precision, use_graph):
+@pytest.mark.parametrize("launch", ["launches", "graph", "persistent"])
+def test_batch_equals_single_runs(gpu_available, precision, launch):
     ps = mixed_instances()
     batch = HydroBatch(ps, precision=precision)
+    assert batch.get_launch() == E2D_BATCH_LAUNCH_PERSISTENT  # the default
+    if launch != "persistent":
+        batch.set_launch(E2D_BATCH_LAUNCH_STEPS)
     batch.set_tile(12)
-    ms, active = batch.run(64, use_graph=use_graph)
+    ms, active = batch.run(64, use_graph=launch == "graph")
     assert active == 0
     for i, p in enumerate(ps):
         run = HydroRun(HydroParams(p.copy()), precision=precision)
@@ -59,6 +63,7 @@ def test_batch_graph_replays_from_any_phase(gpu_available):
         p.nStepmax = 150
         p.tEnd = 1e9
     batch = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
+    batch.set_launch(E2D_BATCH_LAUNCH_STEPS)
     batch.set_tile(12)
     steps = 0
     for k, g in ((7, False), (31, True), (65, True), (2, True)):
@@ -78,13 +83,16 @@ def test_batch_graph_replays_from_any_phase(gpu_available):
 
 @pytest.mark.parametrize("precision", [E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT],
                          ids=["fp64", "split"])
-def test_batch_wide_addressing(gpu_available, precision, monkeypatch):
+@pytest.mark.parametrize("launch", [E2D_BATCH_LAUNCH_STEPS, E2D_BATCH_LAUNCH_PERSISTENT],
+                         ids=["steps", "persistent"])
+def test_batch_wide_addressing(gpu_available, precision, launch, monkeypatch):
     """A batch with the 64-bit row addressing (forced) equals single runs with
     the 32-bit one."""
     ps = mixed_instances()
     monkeypatch.setenv("E2D_WIDE_ADDRESSING", "1")
     batch = HydroBatch(ps, precision=precision)
     monkeypatch.delenv("E2D_WIDE_ADDRESSING")
+    batch.set_launch(launch)
     batch.set_tile(16)
     batch.run(64)
     for i, p in enumerate(ps):
@@ -164,3 +172,62 @@ def test_sweep_instances_accuracy(gpu_available):
           f"worst=sweep[{pick[worst]}] {sel[worst].nx}x{sel[worst].ny} "
           f"problem={sel[worst].problemType} ratio={ratio[pick[worst]]:.0f}", flush=True)
     assert np.all(err <= 1e-4), err
+
+
+@pytest.mark.parametrize("precision", [E2D_PRECISION_FP64, E2D_PRECISION_FP32_SPLIT,
+                                       E2D_PRECISION_FP32], ids=["fp64", "split", "fp32"])
+def test_persistent_launch_equals_single_runs(gpu_available, precision):
+    """All the steps of a run in one persistent launch (per-XCD work queues,
+    per-instance dependency counts) leave every instance where its single run
+    ends, bit for bit: instances that finish early (nStepmax, tEnd) stay put,
+    and two runs in a row continue the control rings' phases."""
+    ps = mixed_instances()
+    batch = HydroBatch(ps, precision=precision)
+    batch.set_tile(12)
+    batch.set_launch(E2D_BATCH_LAUNCH_PERSISTENT)
+    assert batch.get_launch() == E2D_BATCH_LAUNCH_PERSISTENT
+    batch.run(7)
+    ms, active = batch.run(57)
+    assert active == 0 and ms > 0
+    for i, p in enumerate(ps):
+        run = HydroRun(HydroParams(p.copy()), precision=precision)
+        run.run()
+        t, n = batch.status(i)
+        assert (t, n) == (run.t, run.nStep), i
+        np.testing.assert_array_equal(batch.state(i), run.get_state(run.current()),
+                                      err_msg=f"instance {i}")
+
+
+def test_persistent_launch_sweep_equals_step_launches(gpu_available):
+    """64 instances of the seeded sweep, 120 steps: the persistent launch and
+    the per-step launches give the same bits for every instance."""
+    ps = sweep_params(count=64, steps=120)
+    a = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
+    b = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
+    for x in (a, b):
+        x.set_tile(16)
+    a.set_launch(E2D_BATCH_LAUNCH_STEPS)
+    b.set_launch(E2D_BATCH_LAUNCH_PERSISTENT)
+    a.run(120)
+    b.run(120)
+    for i in range(len(ps)):
+        assert a.status(i) == b.status(i), i
+        np.testing.assert_array_equal(a.state(i), b.state(i), err_msg=f"instance {i}")
+
+
+def test_persistent_launch_wait_limit(gpu_available):
+    """A dependency wait bounded to one sleep round runs out on a large
+    instance (its next step's items are taken while its step is still
+    marching): the launch ends, e2d_batch_run reports it, and the batch runs
+    normally again after e2d_batch_init."""
+    from euler2d_kokkos_amd import E2DError
+    ps = [make_params(nx=512, ny=512, problemType=1, cfl=0.7, nStepmax=100, tEnd=1e9)]
+    batch = HydroBatch(ps, precision=E2D_PRECISION_FP64)
+    batch.set_tile(8)
+    batch.set_launch(E2D_BATCH_LAUNCH_PERSISTENT, spin_limit=1)
+    with pytest.raises(E2DError):
+        batch.run(20)
+    batch.set_launch(E2D_BATCH_LAUNCH_STEPS)
+    batch.init()
+    _, active = batch.run(20)
+    assert active == 1 and batch.status(0)[1] == 20
