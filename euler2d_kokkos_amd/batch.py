@@ -1,4 +1,5 @@
-"""Batched driver: many independent instances advanced by one launch per step.
+"""Batched driver: many independent instances advanced together, in one
+persistent launch per run (the default) or one launch per step.
 
 HydroBatch wraps the e2d_batch_* C ABI.  sweep_params() builds the seeded
 512-instance sweep: sizes nx, ny drawn independently from
@@ -6,6 +7,12 @@ HydroBatch wraps the e2d_batch_* C ABI.  sweep_params() builds the seeded
 high-state pressure uniform in [0.5, 2], odd ones blasts with disk radius
 uniform in [0.05, 0.2] and pressure ratio log-uniform in [10, 1e4]
 (p_out = 0.1, rho = 1); gamma 1.4, reflecting walls, HLLC, CFL 0.8.
+
+    python -m euler2d_kokkos_amd.batch [--count 512] [--steps 1000]
+        [--precision split|fp64|fp32] [--launch persistent|steps] [--tile T]
+
+runs the seeded sweep on the GPU and prints one JSON line (cell-updates per
+second over the device time and over the host clock).
 """
 from __future__ import annotations
 
@@ -151,3 +158,48 @@ def density_l1(a: HydroBatch, b: HydroBatch) -> np.ndarray:
         rb = b.state(i, _capi.E2D_LAYOUT_INTERIOR)[0]
         out[i] = np.abs(ra - rb).sum() / np.abs(rb).sum()
     return out
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(prog="python -m euler2d_kokkos_amd.batch",
+                                 description="Runs the seeded instance sweep on the GPU.")
+    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--precision", choices=("split", "fp64", "fp32"), default="split")
+    ap.add_argument("--launch", choices=("persistent", "steps"), default="persistent")
+    ap.add_argument("--tile", type=int, default=0, help="rows per wavefront (0: autotune)")
+    args = ap.parse_args(argv)
+    prec = {"split": _capi.E2D_PRECISION_FP32_SPLIT, "fp64": _capi.E2D_PRECISION_FP64,
+            "fp32": _capi.E2D_PRECISION_FP32}[args.precision]
+    mode = (_capi.E2D_BATCH_LAUNCH_PERSISTENT if args.launch == "persistent"
+            else _capi.E2D_BATCH_LAUNCH_STEPS)
+    ps = sweep_params(count=args.count, steps=args.steps)
+    b = HydroBatch(ps, precision=prec)
+    b.set_launch(mode)
+    tile = args.tile
+    if tile > 0:
+        b.set_tile(tile)
+    else:
+        b.run(60)
+        tile, _ = b.autotune(candidates=(12, 16, 24, 32, 48, 64, 96), steps=60)
+        b.init()
+    t0 = time.perf_counter()
+    ms, active = b.run(args.steps)
+    wall = time.perf_counter() - t0
+    updates = sum(p.nx * p.ny * b.status(i)[1] for i, p in enumerate(ps))
+    print(json.dumps({"instances": len(ps), "steps": args.steps, "precision": args.precision,
+                      "launch": args.launch, "tile": tile,
+                      "device_ms": round(ms, 3),
+                      "gcell_per_s_device": round(updates / (ms * 1e-3) / 1e9, 3),
+                      "gcell_per_s_wall": round(updates / wall / 1e9, 3),
+                      "still_active": active}))
+    b.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
