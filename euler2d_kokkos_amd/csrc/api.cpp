@@ -1127,9 +1127,10 @@ void batch_drop_graph(e2d_batch *b) {
   b->graph_steps = 0;
 }
 
-// default bound of a persistent dependency wait: sleep rounds of about 64
-// cycles, about a second in all
-constexpr unsigned kDefaultSpinLimit = 1u << 25;
+// default bound of a persistent dependency wait, in polls: each is a
+// device-scope load of the completion count (about a microsecond) and a sleep
+// of about 64 cycles, about a second in all
+constexpr unsigned kDefaultSpinLimit = 1u << 20;
 
 // Work queues of the persistent launch for the current tile: instances go to
 // the XCD with the least work so far, largest first (cells as the measure);

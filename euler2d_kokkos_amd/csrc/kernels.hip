@@ -1383,8 +1383,8 @@ void step_batch_kernel_w3(const BatchInst *__restrict__ insts, int ninst, StepCt
 // its CU's L1 lines before reading.  The control rings are read and written
 // with atomics, at the point where the rate atomics are performed.  Items are
 // taken in step-major order, so an item only waits for items already taken
-// by running wavefronts; every wait is bounded all the same (spin_limit sleep
-// rounds), and a wait that runs out sets counters->err and ends the launch.
+// by running wavefronts; every wait is bounded all the same (spin_limit polls
+// of the count), and a wait that runs out sets counters->err and ends the launch.
 
 __device__ __forceinline__ unsigned xcc_id() {
   unsigned x;

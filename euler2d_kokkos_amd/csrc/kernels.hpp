@@ -193,7 +193,7 @@ hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int
 // of 256 threads as the kernel fits on `ncu` CUs at once.  counters and done
 // (one word every kCounterStride per instance) must be zero; steps times the
 // largest per_step must stay below 2^31.
-// spin_limit bounds every dependency wait (sleep rounds); a wait that runs
+// spin_limit bounds every dependency wait (polls); a wait that runs
 // past it sets counters->err and the launch ends early.
 hipError_t launch_step_batch_persistent(const KParams &kp0, const BatchInst *insts_dev, int ninst,
                                         StepCtl *ctls_dev, int ph0, int steps, BatchQueue q,

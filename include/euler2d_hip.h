@@ -308,8 +308,9 @@ int e2d_batch_set_tile(e2d_batch *batch, int seg_rows);
  *                                completed; use_graph is ignored
  *   E2D_BATCH_LAUNCH_STEPS       one launch per step for all instances,
  *                                replayed from hipGraphs when use_graph != 0
- * spin_limit bounds each dependency wait of the persistent launch (sleep
- * rounds; <= 0: the default, about a second); a wait that runs past it ends
+ * spin_limit bounds each dependency wait of the persistent launch (polls of
+ * the completion count, each about a microsecond; <= 0: the default, 2^20,
+ * about a second); a wait that runs past it ends
  * the launch and e2d_batch_run returns E2D_ERR_HIP, leaving the instances
  * part-way (run e2d_batch_init).  Results are the same bits either way. */
 #define E2D_BATCH_LAUNCH_STEPS 0
