@@ -288,7 +288,7 @@ def timed_run(params, precision, steps, tile=0):
     run = HydroRun(params, precision=precision)
     if tile > 0:
         run.set_tile(tile)
-        ms = 1.0
+        ms = run.time_loop(5)[0] / 5  # the settle run's length comes from it
     else:
         _, ms = run.autotune(steps=10)
     settle(run, ms)
@@ -392,6 +392,7 @@ def main() -> int:
     tune_ms = 1.0
     if args.tile > 0:
         run.set_tile(args.tile)
+        tune_ms = run.time_loop(5)[0] / 5  # the settle run's length comes from it
     elif args.calibrate_copy <= 0:
         _, tune_ms = run.autotune(steps=20)
 
