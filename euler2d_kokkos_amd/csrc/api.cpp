@@ -45,6 +45,23 @@ int hip_fail(hipError_t e, const char *what) {
     if (e_ != hipSuccess) return hip_fail(e_, #expr); \
   } while (0)
 
+// Events that time one call on a stream, destroyed on every return path.
+template <int N>
+struct Events {
+  hipEvent_t e[N] = {};
+  hipError_t create() {
+    for (auto &x : e) {
+      const hipError_t rc = hipEventCreate(&x);
+      if (rc != hipSuccess) return rc;
+    }
+    return hipSuccess;
+  }
+  ~Events() {
+    for (auto x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
 // steps launched between two host checks of the loop state
 constexpr int kChunk = 256;
 
@@ -149,9 +166,9 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
   Grid out = (nStep % 2 == 0) ? r->U2 : r->U;
   const double dtdx = dt / r->p.dx;
   const double dtdy = dt / r->p.dy;
-  hipEvent_t ev[5] = {};
-  if (timed)
-    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  Events<5> evs;
+  hipEvent_t *ev = evs.e;
+  if (timed) HIP_TRY(evs.create());
   if (r->schedule != E2D_SCHEDULE_FUSED) {
     if (r->precision != E2D_PRECISION_FP64)
       return fail(E2D_ERR_UNSUPPORTED, "the pass schedules exist for FP64 only");
@@ -220,7 +237,6 @@ int do_step(e2d_run *r, int nStep, double dt, bool timed) {
       HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[4]));
       r->timers[1] += ms * 1e-3;
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
   }
   return E2D_OK;
 }
@@ -351,10 +367,9 @@ int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elap
       HIP_TRY(e2d::launch_boundaries(r->kp, r->U, r->stream));
       HIP_TRY(e2d::launch_boundaries(r->kp, r->U2, r->stream));
     }
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, r->stream));
+    Events<2> ev;
+    HIP_TRY(ev.create());
+    HIP_TRY(hipEventRecord(ev.e[0], r->stream));
     int done = 0;
     while (*t < r->p.tEnd && *nStep < r->p.nStepmax && (max_steps <= 0 || done < max_steps)) {
       double dt;
@@ -367,12 +382,10 @@ int run_loop_impl(e2d_run *r, int max_steps, double *t, int *nStep, double *elap
       *t += dt;
       done++;
     }
-    HIP_TRY(hipEventRecord(e1, r->stream));
-    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventRecord(ev.e[1], r->stream));
+    HIP_TRY(hipEventSynchronize(ev.e[1]));
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     r->timers[1] += ms * 1e-3;
     if (elapsed_ms) *elapsed_ms = ms;
     if (executed) *executed = *nStep - n0;
@@ -1278,10 +1291,9 @@ int batch_run_persistent(e2d_batch *b, int max_steps, double *elapsed_ms, int *a
   long long per_max = 1;
   for (int x = 0; x < b->nxcd; ++x) per_max = std::max(per_max, (long long)b->queue_per_step[x]);
   const int cap = (int)std::min<long long>(INT_MAX, ((1LL << 31) - 1) / per_max);
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, b->stream));
+  Events<2> ev;
+  HIP_TRY(ev.create());
+  HIP_TRY(hipEventRecord(ev.e[0], b->stream));
   unsigned err = 0;
   for (int left = max_steps; left > 0 && !err;) {
     const int steps = std::min(left, cap);
@@ -1301,12 +1313,10 @@ int batch_run_persistent(e2d_batch *b, int max_steps, double *elapsed_ms, int *a
     for (int x = 0; x < b->nxcd && !err; ++x)
       if (c.head[x][0] < (unsigned)steps * (unsigned)b->queue_per_step[x]) err = 2;
   }
-  HIP_TRY(hipEventRecord(e1, b->stream));
-  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventRecord(ev.e[1], b->stream));
+  HIP_TRY(hipEventSynchronize(ev.e[1]));
   float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
   if (err == 2) {
     b->ready = false;
     return fail(E2D_ERR_HIP, "persistent batch launch: an XCD queue was not drained "
@@ -1488,10 +1498,9 @@ int e2d_batch_run(e2d_batch *b, int max_steps, int use_graph, double *elapsed_ms
     rc = batch_graph(b);
     if (rc) return rc;
   }
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, b->stream));
+  Events<2> ev;
+  HIP_TRY(ev.create());
+  HIP_TRY(hipEventRecord(ev.e[0], b->stream));
   int done = 0;
   while (done < max_steps) {
     if (use_graph && b->phase == 0 && max_steps - done >= b->graph_steps) {
@@ -1503,12 +1512,10 @@ int e2d_batch_run(e2d_batch *b, int max_steps, int use_graph, double *elapsed_ms
       done += 1;
     }
   }
-  HIP_TRY(hipEventRecord(e1, b->stream));
-  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventRecord(ev.e[1], b->stream));
+  HIP_TRY(hipEventSynchronize(ev.e[1]));
   float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
   if (elapsed_ms) *elapsed_ms = ms;
   int act = 0;
   rc = batch_active(b, &act);
