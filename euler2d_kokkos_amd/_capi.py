@@ -145,6 +145,7 @@ _PROTOTYPES = {
     "e2d_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
     "e2d_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "e2d_batch_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "e2d_batch_queue_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "e2d_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
                                 C.POINTER(C.c_int)]),
     "e2d_batch_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),

@@ -9,7 +9,7 @@ uniform in [0.05, 0.2] and pressure ratio log-uniform in [10, 1e4]
 (p_out = 0.1, rho = 1); gamma 1.4, reflecting walls, HLLC, CFL 0.8.
 
     python -m euler2d_kokkos_amd.batch [--count 512] [--steps 1000]
-        [--precision split|fp64|fp32] [--launch persistent|steps] [--tile T]
+        [--precision split|fp64|fp32] [--launch auto|persistent|steps] [--tile T]
 
 runs the seeded sweep on the GPU and prints one JSON line (cell-updates per
 second over the device time and over the host clock).
@@ -94,9 +94,10 @@ class HydroBatch:
         check(load_library().e2d_batch_set_tile(self._h, seg_rows))
 
     def set_launch(self, mode: int, spin_limit: int = 0) -> None:
-        """E2D_BATCH_LAUNCH_PERSISTENT (all the steps of a run in one launch,
-        the default) or E2D_BATCH_LAUNCH_STEPS (one launch per step, replayed
-        from hipGraphs); spin_limit bounds each dependency wait of the
+        """E2D_BATCH_LAUNCH_PERSISTENT (all the steps of a run in one launch;
+        the default when the instances fill the XCD queues evenly) or
+        E2D_BATCH_LAUNCH_STEPS (one launch per step, replayed from hipGraphs;
+        the default otherwise); spin_limit bounds each dependency wait of the
         persistent launch."""
         check(load_library().e2d_batch_set_launch(self._h, mode, spin_limit))
 
@@ -104,6 +105,13 @@ class HydroBatch:
         out = C.c_int(-1)
         check(load_library().e2d_batch_get_launch(self._h, C.byref(out)))
         return out.value
+
+    def queue_info(self) -> tuple[int, float]:
+        """(XCD queues of the persistent launch, fullest queue's cells over the mean)."""
+        n = C.c_int(0)
+        imb = C.c_double(0.0)
+        check(load_library().e2d_batch_queue_info(self._h, C.byref(n), C.byref(imb)))
+        return n.value, imb.value
 
     def run(self, steps: int, use_graph: bool = True) -> tuple[float, int]:
         """Runs `steps` batched steps; returns (device ms, instances still active)."""
@@ -170,16 +178,17 @@ def main(argv=None) -> int:
     ap.add_argument("--count", type=int, default=512)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--precision", choices=("split", "fp64", "fp32"), default="split")
-    ap.add_argument("--launch", choices=("persistent", "steps"), default="persistent")
+    ap.add_argument("--launch", choices=("auto", "persistent", "steps"), default="auto",
+                    help="auto: the batch's own default")
     ap.add_argument("--tile", type=int, default=0, help="rows per wavefront (0: autotune)")
     args = ap.parse_args(argv)
     prec = {"split": _capi.E2D_PRECISION_FP32_SPLIT, "fp64": _capi.E2D_PRECISION_FP64,
             "fp32": _capi.E2D_PRECISION_FP32}[args.precision]
-    mode = (_capi.E2D_BATCH_LAUNCH_PERSISTENT if args.launch == "persistent"
-            else _capi.E2D_BATCH_LAUNCH_STEPS)
     ps = sweep_params(count=args.count, steps=args.steps)
     b = HydroBatch(ps, precision=prec)
-    b.set_launch(mode)
+    if args.launch != "auto":
+        b.set_launch(_capi.E2D_BATCH_LAUNCH_PERSISTENT if args.launch == "persistent"
+                     else _capi.E2D_BATCH_LAUNCH_STEPS)
     tile = args.tile
     if tile > 0:
         b.set_tile(tile)
@@ -192,7 +201,8 @@ def main(argv=None) -> int:
     wall = time.perf_counter() - t0
     updates = sum(p.nx * p.ny * b.status(i)[1] for i, p in enumerate(ps))
     print(json.dumps({"instances": len(ps), "steps": args.steps, "precision": args.precision,
-                      "launch": args.launch, "tile": tile,
+                      "launch": ("persistent" if b.get_launch() == _capi.E2D_BATCH_LAUNCH_PERSISTENT
+                                 else "steps"), "tile": tile,
                       "device_ms": round(ms, 3),
                       "gcell_per_s_device": round(updates / (ms * 1e-3) / 1e9, 3),
                       "gcell_per_s_wall": round(updates / wall / 1e9, 3),

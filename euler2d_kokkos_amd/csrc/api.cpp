@@ -1111,6 +1111,7 @@ struct e2d_batch {
   int floor_sqrt = 1;  // every instance allows the short wave-speed square root
   // persistent launch (E2D_BATCH_LAUNCH_PERSISTENT)
   int launch = E2D_BATCH_LAUNCH_PERSISTENT;
+  double queue_imbalance = 1.0;          // cells of the fullest XCD queue / the mean
   unsigned spin_limit = 0;
   int nxcd = 0;                          // XCDs seen by the probe at creation
   int xmap[e2d::kMaxXcds];               // XCC id -> queue (-1: not seen)
@@ -1145,6 +1146,10 @@ void batch_drop_graph(e2d_batch *b) {
 // of about 64 cycles, about a second in all
 constexpr unsigned kDefaultSpinLimit = 1u << 20;
 
+// largest queue imbalance (fullest XCD queue's cells over the mean) at which
+// a new batch defaults to the persistent launch
+constexpr double kPersistentImbalance = 1.10;
+
 // Work queues of the persistent launch for the current tile: instances go to
 // the XCD with the least work so far, largest first (cells as the measure);
 // within an XCD the largest instances come first in each step.
@@ -1174,6 +1179,12 @@ int batch_upload_queue(e2d_batch *b, const std::vector<int> &waves) {
   }
   begin[nx] = (int)order.size();
   b->queue_per_step = per;
+  long long total = 0, most = 0;
+  for (long long l : load) {
+    total += l;
+    most = std::max(most, l);
+  }
+  b->queue_imbalance = total > 0 ? (double)most * nx / (double)total : 1.0;
   std::vector<int> h;
   h.insert(h.end(), order.begin(), order.end());
   h.insert(h.end(), prefix.begin(), prefix.end());
@@ -1355,6 +1366,16 @@ int e2d_batch_get_launch(e2d_batch *b, int *mode) {
   return E2D_OK;
 }
 
+
+int e2d_batch_queue_info(e2d_batch *b, int *nxcd, double *imbalance) {
+  int rc = batch_check(b);
+  if (rc) return rc;
+  if (!nxcd || !imbalance) return fail(E2D_ERR_ARG, "null argument");
+  *nxcd = b->nxcd;
+  *imbalance = b->queue_imbalance;
+  return E2D_OK;
+}
+
 int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_batch **out) {
   if (!params || !out || count <= 0) return fail(E2D_ERR_ARG, "bad argument");
   *out = nullptr;
@@ -1437,6 +1458,11 @@ int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_bat
       e2d_batch_destroy(b);
       return fail(rc, msg);
     }
+    // The persistent launch keeps each instance on one XCD: it is the default
+    // when the instances fill the XCD queues evenly, and one launch per step
+    // (every instance over the whole GPU) otherwise, e.g. for a few large grids.
+    b->launch = b->queue_imbalance <= kPersistentImbalance ? E2D_BATCH_LAUNCH_PERSISTENT
+                                                           : E2D_BATCH_LAUNCH_STEPS;
     *out = b;
     return E2D_OK;
   });

@@ -300,14 +300,18 @@ int e2d_batch_init(e2d_batch *batch);
 int e2d_batch_set_tile(e2d_batch *batch, int seg_rows);
 /* How e2d_batch_run launches the steps (no reference counterpart: the
  * reference runs one grid per process, src/main.cpp:86).
- *   E2D_BATCH_LAUNCH_PERSISTENT  (the default) all the run's steps in one
- *                                launch: every instance is assigned to one
- *                                XCD, whose wavefronts take its work items
- *                                from that XCD's queue and start its step
- *                                n + 1 as soon as its own step n has
- *                                completed; use_graph is ignored
+ *   E2D_BATCH_LAUNCH_PERSISTENT  all the run's steps in one launch: every
+ *                                instance is assigned to one XCD, whose
+ *                                wavefronts take its work items from that
+ *                                XCD's queue and start its step n + 1 as soon
+ *                                as its own step n has completed; use_graph
+ *                                is ignored
  *   E2D_BATCH_LAUNCH_STEPS       one launch per step for all instances,
  *                                replayed from hipGraphs when use_graph != 0
+ * A new batch takes the persistent launch when its instances fill the XCD
+ * queues evenly (the fullest queue holds at most 10% more cells than the
+ * mean), else one launch per step (a few large grids would otherwise leave
+ * XCDs idle); e2d_batch_get_launch reports the choice.
  * spin_limit bounds each dependency wait of the persistent launch (polls of
  * the completion count, each about a microsecond; <= 0: the default, 2^20,
  * about a second); a wait that runs past it ends
@@ -317,6 +321,10 @@ int e2d_batch_set_tile(e2d_batch *batch, int seg_rows);
 #define E2D_BATCH_LAUNCH_PERSISTENT 1
 int e2d_batch_set_launch(e2d_batch *batch, int mode, int spin_limit);
 int e2d_batch_get_launch(e2d_batch *batch, int *mode);
+/* XCD queues of the persistent launch (one per XCD the device showed at
+ * e2d_batch_create) and their imbalance: the fullest queue's cells over the
+ * mean (1.0 = even). */
+int e2d_batch_queue_info(e2d_batch *batch, int *nxcd, double *imbalance);
 /* Runs up to max_steps batched steps (instances whose loop has ended stay put).
  * use_graph != 0 replays captured hipGraphs of the step launch.  Returns the
  * device time in milliseconds and the number of instances still active.

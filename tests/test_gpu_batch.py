@@ -37,9 +37,8 @@ def mixed_instances():
 def test_batch_equals_single_runs(gpu_available, precision, launch):
     ps = mixed_instances()
     batch = HydroBatch(ps, precision=precision)
-    assert batch.get_launch() == E2D_BATCH_LAUNCH_PERSISTENT  # the default
-    if launch != "persistent":
-        batch.set_launch(E2D_BATCH_LAUNCH_STEPS)
+    batch.set_launch(E2D_BATCH_LAUNCH_PERSISTENT if launch == "persistent"
+                     else E2D_BATCH_LAUNCH_STEPS)
     batch.set_tile(12)
     ms, active = batch.run(64, use_graph=launch == "graph")
     assert active == 0
@@ -231,3 +230,23 @@ def test_persistent_launch_wait_limit(gpu_available):
     batch.init()
     _, active = batch.run(20)
     assert active == 1 and batch.status(0)[1] == 20
+
+
+def test_default_launch_follows_queue_balance(gpu_available):
+    """The seeded 512-instance sweep fills the XCD queues evenly and takes the
+    persistent launch; six grids of different sizes, or one grid, would leave
+    XCDs idle and take one launch per step."""
+    b = HydroBatch(sweep_params(steps=10), precision=E2D_PRECISION_FP32_SPLIT)
+    nxcd, imbalance = b.queue_info()
+    print(f"XCD queues {nxcd}, sweep imbalance {imbalance:.4f}", flush=True)
+    assert nxcd >= 1 and 1.0 <= imbalance <= 1.10
+    assert b.get_launch() == E2D_BATCH_LAUNCH_PERSISTENT
+    b.close()
+    for ps in (mixed_instances(), mixed_instances()[:1]):
+        b = HydroBatch(ps, precision=E2D_PRECISION_FP64)
+        n, imb = b.queue_info()
+        expect = E2D_BATCH_LAUNCH_PERSISTENT if imb <= 1.10 else E2D_BATCH_LAUNCH_STEPS
+        assert b.get_launch() == expect, (n, imb)
+        if n > 1 and len(ps) == 1:  # one grid on one of several XCDs
+            assert b.get_launch() == E2D_BATCH_LAUNCH_STEPS and imb == n
+        b.close()
