@@ -102,9 +102,12 @@ def test_batch_wide_addressing(gpu_available, precision, launch, monkeypatch):
                                       err_msg=f"instance {i}")
 
 
-def test_batch_fp64_equals_oracle(gpu_available):
+@pytest.mark.parametrize("launch", [E2D_BATCH_LAUNCH_STEPS, E2D_BATCH_LAUNCH_PERSISTENT],
+                         ids=["steps", "persistent"])
+def test_batch_fp64_equals_oracle(gpu_available, launch):
     ps = mixed_instances()[:3]
     batch = HydroBatch(ps, precision=E2D_PRECISION_FP64)
+    batch.set_launch(launch)
     batch.run(40)
     for i, p in enumerate(ps):
         ora = O.OracleRun(p)
