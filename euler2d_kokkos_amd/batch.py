@@ -145,6 +145,9 @@ class HydroBatch:
         return {"cells": cells.value, "waves": waves.value, "count": count.value}
 
     def autotune(self, candidates=(8, 10, 12, 14, 16, 20, 24, 32), steps: int = 5):
+        """Times the candidate tiles (and, until set_launch fixes it, both launch
+        modes where the queues allow the persistent one); keeps the fastest and
+        returns (tile, ms per step)."""
         cand = (C.c_int * len(candidates))(*candidates)
         best = C.c_int(0)
         ms = C.c_double(0.0)

@@ -1112,6 +1112,7 @@ struct e2d_batch {
   // persistent launch (E2D_BATCH_LAUNCH_PERSISTENT)
   int launch = E2D_BATCH_LAUNCH_PERSISTENT;
   double queue_imbalance = 1.0;          // cells of the fullest XCD queue / the mean
+  bool launch_chosen = false;            // e2d_batch_set_launch has fixed the mode
   unsigned spin_limit = 0;
   int nxcd = 0;                          // XCDs seen by the probe at creation
   int xmap[e2d::kMaxXcds];               // XCC id -> queue (-1: not seen)
@@ -1356,6 +1357,7 @@ int e2d_batch_set_launch(e2d_batch *b, int mode, int spin_limit) {
   if (mode != E2D_BATCH_LAUNCH_STEPS && mode != E2D_BATCH_LAUNCH_PERSISTENT)
     return fail(E2D_ERR_ARG, "mode must be E2D_BATCH_LAUNCH_STEPS or _PERSISTENT");
   b->launch = mode;
+  b->launch_chosen = true;
   b->spin_limit = spin_limit > 0 ? (unsigned)spin_limit : 0u;
   return E2D_OK;
 }
@@ -1585,35 +1587,48 @@ int e2d_batch_autotune(e2d_batch *b, const int *candidates, int ncand, int steps
   if (rc) return rc;
   if (!candidates || ncand <= 0 || steps <= 0) return fail(E2D_ERR_ARG, "bad argument");
   if (!b->ready) return fail(E2D_ERR_ARG, "e2d_batch_init has not run on this batch");
-  int best = b->seg_rows;
-  double best_ms = 1e300;
-  // two interleaved passes; the faster timing of each candidate counts
-  std::vector<double> cand_ms((size_t)ncand, 1e300);
+  // the launch modes to try: the one e2d_batch_set_launch fixed, else both
+  // where the queues are even enough for the persistent launch
+  std::vector<int> modes{b->launch};
+  if (!b->launch_chosen && b->queue_imbalance <= kPersistentImbalance)
+    modes = {E2D_BATCH_LAUNCH_PERSISTENT, E2D_BATCH_LAUNCH_STEPS};
+  const int nm = (int)modes.size();
+  // two interleaved passes; the faster timing of each (mode, tile) counts
+  std::vector<double> cand_ms((size_t)ncand * nm, 1e300);
   for (int pass = 0; pass < 2; ++pass) {
-    for (int c = 0; c < ncand; ++c) {
-      rc = e2d_batch_set_tile(b, candidates[c]);
-      if (rc) return rc;
-      double ms = 0.0;
-      int act = 0;
-      for (int attempt = 0; attempt < 2; ++attempt) {
-        rc = e2d_batch_run(b, 2, 0, &ms, &act);  // warm-up
+    for (int m = 0; m < nm; ++m) {
+      b->launch = modes[(size_t)m];
+      for (int c = 0; c < ncand; ++c) {
+        rc = e2d_batch_set_tile(b, candidates[c]);
         if (rc) return rc;
-        rc = e2d_batch_run(b, steps, 0, &ms, &act);
-        if (rc) return rc;
-        // a run in which instances reached the end of their loops timed
-        // launches that returned at once: start the batch again and repeat it
-        if (act == (int)b->runs.size() || attempt == 1) break;
-        rc = e2d_batch_init(b);
-        if (rc) return rc;
+        double ms = 0.0;
+        int act = 0;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+          rc = e2d_batch_run(b, 2, 0, &ms, &act);  // warm-up
+          if (rc) return rc;
+          rc = e2d_batch_run(b, steps, 0, &ms, &act);
+          if (rc) return rc;
+          // a run in which instances reached the end of their loops timed
+          // launches that returned at once: start the batch again and repeat it
+          if (act == (int)b->runs.size() || attempt == 1) break;
+          rc = e2d_batch_init(b);
+          if (rc) return rc;
+        }
+        double &slot = cand_ms[(size_t)m * ncand + c];
+        slot = std::min(slot, ms / steps);
       }
-      cand_ms[(size_t)c] = std::min(cand_ms[(size_t)c], ms / steps);
     }
   }
-  for (int c = 0; c < ncand; ++c)
-    if (cand_ms[(size_t)c] < best_ms) {
-      best_ms = cand_ms[(size_t)c];
-      best = candidates[c];
-    }
+  int best = b->seg_rows, best_mode = modes[0];
+  double best_ms = 1e300;
+  for (int m = 0; m < nm; ++m)
+    for (int c = 0; c < ncand; ++c)
+      if (cand_ms[(size_t)m * ncand + c] < best_ms) {
+        best_ms = cand_ms[(size_t)m * ncand + c];
+        best = candidates[c];
+        best_mode = modes[(size_t)m];
+      }
+  b->launch = best_mode;
   rc = e2d_batch_set_tile(b, best);
   if (rc) return rc;
   if (best_seg_rows) *best_seg_rows = best;

@@ -336,11 +336,15 @@ int e2d_batch_status(e2d_batch *batch, int instance, double *t, int *nStep);
 int e2d_batch_copy_state(e2d_batch *batch, int instance, double *host, int layout);
 /* total interior cells, wavefronts per step, instance count */
 int e2d_batch_info(e2d_batch *batch, long long *cells, int *waves, int *count);
-/* Times each candidate seg_rows over `steps` batched steps in the batch's
- * launch mode, in two interleaved passes, and keeps the fastest; returns it.  A candidate whose run
- * saw instances reach the end of their loops is timed again after
- * e2d_batch_init.  Restoring the state afterwards is the caller's business:
- * call e2d_batch_init. */
+/* Times each candidate seg_rows over `steps` batched steps, in two
+ * interleaved passes, and keeps the fastest; returns it.  Until
+ * e2d_batch_set_launch fixes the launch mode, and when the instances fill the
+ * XCD queues evenly, each candidate is timed in both launch modes and the
+ * fastest (mode, seg_rows) pair is kept (e2d_batch_get_launch reports the
+ * mode); otherwise the batch's launch mode is used.  A candidate whose run saw
+ * instances reach the end of their loops is timed again after e2d_batch_init.
+ * Restoring the state afterwards is the caller's business: call
+ * e2d_batch_init. */
 int e2d_batch_autotune(e2d_batch *batch, const int *candidates, int ncand, int steps,
                        int *best_seg_rows, double *best_ms_per_step);
 

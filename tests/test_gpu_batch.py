@@ -253,3 +253,25 @@ def test_default_launch_follows_queue_balance(gpu_available):
         if n > 1 and len(ps) == 1:  # one grid on one of several XCDs
             assert b.get_launch() == E2D_BATCH_LAUNCH_STEPS and imb == n
         b.close()
+
+
+def test_autotune_picks_launch_mode_and_tile(gpu_available):
+    """With the launch mode left to the batch and even queues, the autotuner
+    times both modes and keeps the fastest (mode, tile) pair; a mode fixed by
+    set_launch stays.  Either way the batch then runs to the same bits."""
+    ps = sweep_params(count=64, steps=40)
+    a = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
+    assert a.queue_info()[1] <= 1.10
+    tile, ms = a.autotune(candidates=(16, 24), steps=3)
+    assert tile in (16, 24) and ms > 0
+    assert a.get_launch() in (E2D_BATCH_LAUNCH_PERSISTENT, E2D_BATCH_LAUNCH_STEPS)
+    b = HydroBatch(ps, precision=E2D_PRECISION_FP32_SPLIT)
+    b.set_launch(E2D_BATCH_LAUNCH_STEPS)
+    b.autotune(candidates=(16, 24), steps=3)
+    assert b.get_launch() == E2D_BATCH_LAUNCH_STEPS
+    for x in (a, b):
+        x.init()
+        x.run(40)
+    for i in range(len(ps)):
+        assert a.status(i) == b.status(i), i
+        np.testing.assert_array_equal(a.state(i), b.state(i), err_msg=f"instance {i}")
