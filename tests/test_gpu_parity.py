@@ -277,6 +277,23 @@ def test_autotune_restores_state(gpu_available):
     _compare(run, ora, "autotuned")
 
 
+def test_autotune_timing_matches_time_loop(gpu_available):
+    """A candidate's ms per step from the autotuner (60-step runs, whose loop
+    graphs are captured before the timed region) agrees with time_loop's on the
+    same tile after a warm-up, within the spread of the engine clock."""
+    p = O.default_params(nx=1024, ny=1536, problemType=1, cfl=0.7, nStepmax=10**6, tEnd=1e9,
+                         ymax=1.5, blast_center_y=0.75)
+    run = HydroRun(HydroParams(p.copy()))
+    run.time_loop(200)  # the engine clock settles
+    best, ms_tune = run.autotune(candidates=(14,), steps=60)
+    assert best == 14
+    run.time_loop(60)
+    ms, n = run.time_loop(60)
+    print(f"autotune {ms_tune * 1e3:.2f} us/step, time_loop {ms / n * 1e3:.2f} us/step",
+          flush=True)
+    assert n == 60 and 0.8 <= ms_tune / (ms / n) <= 1.25
+
+
 @pytest.mark.parametrize("case", ["implode64", "blast48x72", "four_quadrant64_bc2",
                                   "periodic_blast", "four_quadrant_hll"])
 def test_atomic_schedule_l1(gpu_available, case):
