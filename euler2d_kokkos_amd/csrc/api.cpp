@@ -86,9 +86,9 @@ struct e2d_run {
   e2d::StepState *host_s = nullptr;  // pinned copy of ctl->s, written by the step kernels
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // the run loop's timing events
   int phase = 0;  // phase of the next run-loop launch in the control rings
-  // a captured run of kLoopGraphSteps run-loop launches (for seg_rows = loop_graph_tile)
-  hipGraphExec_t loop_graph = nullptr;
-  int loop_graph_tile = 0;
+  // captured runs of kLoopGraphSteps[g] run-loop launches (for seg_rows = loop_graph_tile[g])
+  hipGraphExec_t loop_graph[2] = {nullptr, nullptr};
+  int loop_graph_tile[2] = {0, 0};
   unsigned long long *bits = nullptr;
   unsigned int *count = nullptr;
   double *partials = nullptr;
@@ -255,21 +255,24 @@ int loop_begin(e2d_run *r, double t, int nStep) {
   return E2D_OK;
 }
 
-// Run-loop launches per captured graph: a multiple of 6, so that a replay that
+// Run-loop launches per captured graph: multiples of 6, so that a replay that
 // starts at ring phase 0 on an even step ends there (phase ring 3, array
-// parity 2).  Budgets below it run as single launches: between two graph
-// launches the GPU idles about 8.6 us, between two single launches it does
-// not (a 20-step budget ran 1.5% faster than as three 6-launch graphs and two
+// parity 2), the long one below the launches between two host checks of an
+// open-ended loop.  Between two graph launches the GPU idles about 8.6 us,
+// between two single launches it does not: the long graph carries long loops
+// with few such gaps, and budgets below the short one run as single launches
+// (a 20-step budget ran 1.5% faster than as three 6-launch graphs and two
 // single launches).
-constexpr int kLoopGraphSteps = 30;
+constexpr int kLoopGraphSteps[2] = {240, 30};
+static_assert(kLoopGraphSteps[0] <= kChunk, "the long loop graph must fit in one chunk");
 
-int loop_graph(e2d_run *r) {
-  if (r->loop_graph && r->loop_graph_tile == r->seg_rows) return E2D_OK;
-  if (r->loop_graph) (void)hipGraphExecDestroy(r->loop_graph);
-  r->loop_graph = nullptr;
+int loop_graph(e2d_run *r, int g) {
+  if (r->loop_graph[g] && r->loop_graph_tile[g] == r->seg_rows) return E2D_OK;
+  if (r->loop_graph[g]) (void)hipGraphExecDestroy(r->loop_graph[g]);
+  r->loop_graph[g] = nullptr;
   hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < kLoopGraphSteps; ++k) {
+  for (int k = 0; k < kLoopGraphSteps[g]; ++k) {
     const Grid &in = (k % 2 == 0) ? r->U : r->U2;
     const Grid &out = (k % 2 == 0) ? r->U2 : r->U;
     hipError_t e = e2d::launch_step_fused(r->kp, in, out, 0.0, r->ctl, k % 3, r->seg_rows,
@@ -281,34 +284,41 @@ int loop_graph(e2d_run *r) {
     }
   }
   HIP_TRY(hipStreamEndCapture(r->stream, &graph));
-  hipError_t e = hipGraphInstantiate(&r->loop_graph, graph, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&r->loop_graph[g], graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
-    r->loop_graph = nullptr;
+    r->loop_graph[g] = nullptr;
     return hip_fail(e, "hipGraphInstantiate");
   }
-  r->loop_graph_tile = r->seg_rows;
+  r->loop_graph_tile[g] = r->seg_rows;
   return E2D_OK;
 }
 
-// Captures, ahead of a timed region, the graph a loop of `count` launches will
-// replay (capture and instantiation are host work of about a millisecond).
+// Captures, ahead of a timed region, the graphs a loop of `count` launches
+// will replay (capture and instantiation are host work of about a millisecond
+// per 30 launches).
 int loop_graphs_for(e2d_run *r, int count) {
-  return count >= kLoopGraphSteps ? loop_graph(r) : E2D_OK;
+  for (int g = 0; g < 2; ++g)
+    if (count >= kLoopGraphSteps[g]) {
+      int rc = loop_graph(r, g);
+      if (rc) return rc;
+    }
+  return E2D_OK;
 }
 
-// Queues `count` run-loop launches from step nStep0 on: captured runs of 30
-// launches where the ring phase is 0 and the step even, single launches
+// Queues `count` run-loop launches from step nStep0 on: captured runs of 240
+// or 30 launches where the ring phase is 0 and the step even, single launches
 // elsewhere.
 int loop_steps(e2d_run *r, int nStep0, int count) {
   int k = 0;
   while (k < count) {
     const int n = nStep0 + k;
-    if (r->phase == 0 && n % 2 == 0 && count - k >= kLoopGraphSteps) {
-      int rc = loop_graph(r);
+    if (r->phase == 0 && n % 2 == 0 && count - k >= kLoopGraphSteps[1]) {
+      const int g = count - k >= kLoopGraphSteps[0] ? 0 : 1;
+      int rc = loop_graph(r, g);
       if (rc) return rc;
-      HIP_TRY(hipGraphLaunch(r->loop_graph, r->stream));
-      k += kLoopGraphSteps;
+      HIP_TRY(hipGraphLaunch(r->loop_graph[g], r->stream));
+      k += kLoopGraphSteps[g];
       continue;
     }
     Grid in = (n % 2 == 0) ? r->U : r->U2;
@@ -740,7 +750,8 @@ int e2d_destroy(e2d_run *run) {
   if (!run) return fail(E2D_ERR_ARG, "null e2d_run handle");
   (void)hipSetDevice(run->device);
   if (run->stream) (void)hipStreamSynchronize(run->stream);
-  if (run->loop_graph) (void)hipGraphExecDestroy(run->loop_graph);
+  for (hipGraphExec_t g : run->loop_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   if (run->ev0) (void)hipEventDestroy(run->ev0);
   if (run->ev1) (void)hipEventDestroy(run->ev1);
   if (run->host_s) (void)hipHostFree(run->host_s);
