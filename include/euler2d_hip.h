@@ -288,9 +288,10 @@ int e2d_write_npy(const char *path, const double *data, int64_t n);
 /* Many independent problems advanced together (no reference counterpart: the
  * reference runs one grid per process, src/main.cpp:86).  Every instance keeps
  * the HydroRun semantics above -- its own grid size, problem, dt, t, nStep --
- * and all share the precision, the Riemann solver and the step kernel: one
- * launch advances every active instance by one step, and runs of steps are
- * replayed from a captured hipGraph. */
+ * and all share the precision, the Riemann solver and the step kernel.  A run
+ * is one persistent launch (each instance steps as soon as its own previous
+ * step is done) or one launch per step for all instances, replayed from a
+ * captured hipGraph: see e2d_batch_set_launch. */
 typedef struct e2d_batch e2d_batch;
 int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_batch **out);
 int e2d_batch_destroy(e2d_batch *batch);
