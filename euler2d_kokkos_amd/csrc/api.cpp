@@ -1114,6 +1114,7 @@ struct e2d_batch {
   // persistent launch (E2D_BATCH_LAUNCH_PERSISTENT)
   int launch = E2D_BATCH_LAUNCH_PERSISTENT;
   double queue_imbalance = 1.0;          // cells of the fullest XCD queue / the mean
+  long long max_items = (1LL << 31) - 1;  // work items of one persistent launch, at most
   bool launch_chosen = false;            // e2d_batch_set_launch has fixed the mode
   unsigned spin_limit = 0;
   int nxcd = 0;                          // XCDs seen by the probe at creation
@@ -1304,7 +1305,7 @@ int batch_run_persistent(e2d_batch *b, int max_steps, double *elapsed_ms, int *a
   unsigned *done = reinterpret_cast<unsigned *>(b->counters + 1);
   long long per_max = 1;
   for (int x = 0; x < b->nxcd; ++x) per_max = std::max(per_max, (long long)b->queue_per_step[x]);
-  const int cap = (int)std::min<long long>(INT_MAX, ((1LL << 31) - 1) / per_max);
+  const int cap = (int)std::max<long long>(1, std::min<long long>(INT_MAX, b->max_items / per_max));
   Events<2> ev;
   HIP_TRY(ev.create());
   HIP_TRY(hipEventRecord(ev.e[0], b->stream));
@@ -1461,6 +1462,13 @@ int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_bat
       const std::string msg = g_last_error;
       e2d_batch_destroy(b);
       return fail(rc, msg);
+    }
+    // E2D_PERSISTENT_MAX_ITEMS lowers the item count of one persistent launch
+    // (below 2^31 anyway): a run then takes several launches, which tests use
+    const char *mi = std::getenv("E2D_PERSISTENT_MAX_ITEMS");
+    if (mi && *mi) {
+      const long long v = std::atoll(mi);
+      if (v > 0 && v < b->max_items) b->max_items = v;
     }
     // The persistent launch keeps each instance on one XCD: it is the default
     // when the instances fill the XCD queues evenly, and one launch per step

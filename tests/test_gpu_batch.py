@@ -275,3 +275,30 @@ def test_autotune_picks_launch_mode_and_tile(gpu_available):
     for i in range(len(ps)):
         assert a.status(i) == b.status(i), i
         np.testing.assert_array_equal(a.state(i), b.state(i), err_msg=f"instance {i}")
+
+
+def test_persistent_run_split_over_launches(gpu_available, monkeypatch):
+    """A run whose item count passes the per-launch limit (lowered here with
+    E2D_PERSISTENT_MAX_ITEMS to 500 items)
+    takes several persistent launches; the control rings' phases continue
+    across them, and the instances end where per-step launches leave them."""
+    ps = sweep_params(count=16, steps=50)
+    a = HydroBatch(ps, precision=E2D_PRECISION_FP64)
+    a.set_launch(E2D_BATCH_LAUNCH_STEPS)
+    monkeypatch.setenv("E2D_PERSISTENT_MAX_ITEMS", "500")
+    b = HydroBatch(ps, precision=E2D_PRECISION_FP64)
+    monkeypatch.delenv("E2D_PERSISTENT_MAX_ITEMS")
+    b.set_launch(E2D_BATCH_LAUNCH_PERSISTENT)
+    for x in (a, b):
+        x.set_tile(12)
+    # the fullest XCD queue holds at least waves / nxcd items per step, so a
+    # launch covers at most 500 * nxcd / waves steps: fewer than 11
+    nxcd, _ = b.queue_info()
+    assert b.info()["waves"] * 11 > 500 * nxcd
+    a.run(11)
+    a.run(39)
+    b.run(11)
+    b.run(39)
+    for i in range(len(ps)):
+        assert a.status(i) == b.status(i), i
+        np.testing.assert_array_equal(a.state(i), b.state(i), err_msg=f"instance {i}")
