@@ -236,7 +236,7 @@ struct PSplit {
     prim32(kp, c, q);
     if (kinetic(kp, c, q)) {
       Q4 q64;
-      prim_no_c(kp.s, value(c, 0), value(c, 1), value(c, 2), value(c, 3), q64);
+      prim_no_c<double, false>(kp.s, value(c, 0), value(c, 1), value(c, 2), value(c, 3), q64);
       for (int v = 0; v < 4; v++) q.v[v] = (float)q64.v[v];
     }
   }
@@ -326,6 +326,14 @@ struct P32 {
     return o;
   }
 };
+
+// CFL rate of one cell (ComputeDtFunctor body)
+template <class P>
+__device__ __forceinline__ typename P::T cell_rate(const KParams &kp, const typename P::Cell &c) {
+  Q4T<typename P::T> q;
+  P::prim(kp, c, q);
+  return cfl_rate_q(P::scheme(kp), q, P::rate(kp));
+}
 
 __device__ __forceinline__ double &at(const Grid &g, int v, int i, int j) {
   return static_cast<double *>(g.p)[v * g.plane + (long long)j * g.pitch + i];
@@ -518,11 +526,8 @@ __global__ __launch_bounds__(256) void dt_reduce_kernel(KParams kp, Grid U,
   T m = T(0.0);
   // block b: interior rows 2+b, 2+b+G, ...; threads stride along the row
   for (int j = 2 + blockIdx.x; j <= kp.ny + 1; j += gridDim.x)
-    for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x) {
-      Q4T<T> q;
-      P::prim(kp, P::load(U, (long long)j * U.pitch + i), q);
-      m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, P::rate(kp)));
-    }
+    for (int i = 2 + threadIdx.x; i <= kp.nx + 1; i += blockDim.x)
+      m = Ops<T>::mx(m, cell_rate<P>(kp, P::load(U, (long long)j * U.pitch + i)));
   // block max, then one device-scope atomic per block (the max is exact and
   // order independent)
   __shared__ double wm[256 / kLanes];
@@ -607,11 +612,7 @@ __global__ __launch_bounds__(kBeginThreads) void loop_begin_kernel(KParams kp, G
       }
 #pragma unroll
       for (int u = 0; u < kRows; u++) {
-        if (j0 + u * (int)gridDim.x <= kp.ny + 1) {
-          Q4T<T> q;
-          P::prim(kp, c[u], q);
-          m = Ops<T>::mx(m, cfl_rate_q(P::scheme(kp), q, P::rate(kp)));
-        }
+        if (j0 + u * (int)gridDim.x <= kp.ny + 1) m = Ops<T>::mx(m, cell_rate<P>(kp, c[u]));
       }
     }
   __shared__ double wm[kBeginThreads / kLanes];
@@ -1016,11 +1017,8 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     const int b = r - 2;
     if (out_col && (!ROWS || b <= kp.ny + 1)) {
       P::template store_rc<O32>(out, b, pin_offset((unsigned)col * P::kElem), unew);
-      if constexpr (RATE) {  // next-step CFL rate of the new state
-        Q4T<T> qn;
-        P::prim(kp, unew, qn);
-        m.wmax = Ops<T>::mx(m.wmax, cfl_rate_q(s, qn, P::rate(kp)));
-      }
+      if constexpr (RATE)  // next-step CFL rate of the new state
+        m.wmax = Ops<T>::mx(m.wmax, cell_rate<P>(kp, unew));
     }
   }
   if constexpr ((STAGES & kYFace) != 0) {

@@ -103,22 +103,94 @@ struct Q4T {
 };
 using Q4 = Q4T<double>;
 
-// computePrimitives, HydroBaseFunctor.h:76-102 (the sound speed is returned
-// separately because only the dt reduction needs it)
-template <class T>
-__device__ __forceinline__ void prim_no_c(const SchemeT<T> &s, T u0, T u1, T u2, T u3,
-                                          Q4T<T> &q) {
-  using O = Ops<T>;
-  const T d = O::mx(u0, s.smallr);
-  const T ux = O::div(u2, d);
-  const T uy = O::div(u3, d);
-  const T eken = T(0.5) * O::mad(uy, uy, ux * ux);
-  const T e = O::div(u1, d) - eken;
-  const T p = O::mx((s.gamma0 - T(1.0)) * d * e, d * s.smallp);
+// ------------------------------------------------ FP64 quotients sharing d
+// The compiler's IEEE FP64 division a / d is div_scale(d), rcp, two Newton
+// steps, m = a r, e = a - d m, q = m + e r (div_fmas), div_fixup.  For the
+// quotients of one cell by its density the reciprocal is formed once, by the
+// same operations, and each quotient takes the three steps after it.  The
+// scaling and fix-up steps act only where d or 1/d is not normal, where the
+// residual e is denormal (a below about 2^-969), or where q is not normal: the
+// class tests below catch each of these, and a wavefront with any such lane
+// takes the IEEE division for all of its lanes.  A zero quotient takes the
+// sign of m, as div_fixup gives it.  Same bits; 27 instead of 33 VALU
+// instructions for the three quotients of prim.  Each fallback is a branch
+// that splits the march's steady row; sharing the reciprocal also with the
+// sound speed's quotient of the CFL rate (a third branch per row) measured
+// slower on the 1024x1536 step, so that quotient stays an IEEE division.
+
+__device__ __forceinline__ double recip_newton(double d) {
+  const double r0 = __builtin_amdgcn_rcp(d);
+  const double r1 = __builtin_fma(r0, __builtin_fma(-d, r0, 1.0), r0);
+  return __builtin_fma(r1, __builtin_fma(-d, r1, 1.0), r1);
+}
+
+constexpr int kClassNormal = (1 << 3) | (1 << 8);
+constexpr int kClassZeroOrNormal = kClassNormal | (1 << 5) | (1 << 6);
+
+__device__ __forceinline__ double quot_shared(double a, double d, double r, bool &ok) {
+  const double m = a * r;
+  const double e = __builtin_fma(-d, m, a);
+  const double q = __builtin_fma(e, r, m);
+  ok = ok && __builtin_amdgcn_class(q, kClassZeroOrNormal) &&
+       __builtin_amdgcn_class(e, kClassZeroOrNormal);
+  return __builtin_copysign(q, m);
+}
+
+// true in every lane when no active lane of the wavefront has !ok
+__device__ __forceinline__ bool wave_all(bool ok) {
+  return __builtin_amdgcn_ballot_w64(!ok) == 0;
+}
+
+// computePrimitives for FP64 with the reciprocal of d kept: r and whether
+// quotients by d formed from it are the IEEE ones (d and r normal)
+__device__ __forceinline__ void prim_fp64(const SchemeT<double> &s, double u0, double u1,
+                                          double u2, double u3, Q4T<double> &q, double &r,
+                                          bool &rok) {
+  const double d = fmax(u0, s.smallr);
+  r = recip_newton(d);
+  rok = __builtin_amdgcn_class(d, kClassNormal) && __builtin_amdgcn_class(r, kClassNormal);
+  bool ok = rok;
+  double ux = quot_shared(u2, d, r, ok);
+  double uy = quot_shared(u3, d, r, ok);
+  double e1 = quot_shared(u1, d, r, ok);
+  if (!wave_all(ok)) {  // uniform
+    ux = u2 / d;
+    uy = u3 / d;
+    e1 = u1 / d;
+  }
+  const double eken = 0.5 * (uy * uy + ux * ux);
+  const double e = e1 - eken;
+  const double p = fmax((s.gamma0 - 1.0) * d * e, d * s.smallp);
   q.v[ID] = d;
   q.v[IP] = p;
   q.v[IU] = ux;
   q.v[IV] = uy;
+}
+
+// computePrimitives, HydroBaseFunctor.h:76-102 (the sound speed is returned
+// separately because only the dt reduction needs it).  SHARED = false keeps
+// the three IEEE divisions for FP64 as well (fewer registers: the split
+// step's FP64 fallback for kinetic-dominated cells).
+template <class T, bool SHARED = true>
+__device__ __forceinline__ void prim_no_c(const SchemeT<T> &s, T u0, T u1, T u2, T u3,
+                                          Q4T<T> &q) {
+  using O = Ops<T>;
+  if constexpr (SHARED && sizeof(T) == sizeof(double)) {
+    double r;
+    bool rok;
+    prim_fp64(s, u0, u1, u2, u3, q, r, rok);
+  } else {
+    const T d = O::mx(u0, s.smallr);
+    const T ux = O::div(u2, d);
+    const T uy = O::div(u3, d);
+    const T eken = T(0.5) * O::mad(uy, uy, ux * ux);
+    const T e = O::div(u1, d) - eken;
+    const T p = O::mx((s.gamma0 - T(1.0)) * d * e, d * s.smallp);
+    q.v[ID] = d;
+    q.v[IP] = p;
+    q.v[IU] = ux;
+    q.v[IV] = uy;
+  }
 }
 
 // ComputeDtFunctor body, HydroRunFunctors.h:53-74: (c+|u|)/dx + (c+|v|)/dy
