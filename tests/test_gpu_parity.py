@@ -16,6 +16,7 @@ from euler2d_kokkos_amd import (E2D_LAYOUT_INTERIOR, E2D_LAYOUT_LEFT, E2D_SCHEDU
 pytestmark = pytest.mark.gpu
 
 RTOL_L1 = 1e-12
+IU, IV = 2, 3  # momentum planes of the [i][j][var] arrays (ID 0, IP 1: the energy)
 
 CASES = {
     "implode64": dict(nx=64, ny=64, problemType=0, cfl=0.8, nStepmax=60, tEnd=1e9),
@@ -275,6 +276,35 @@ def test_autotune_restores_state(gpu_available):
     run.run()
     ora.run()
     _compare(run, ora, "autotuned")
+
+
+@pytest.mark.parametrize("schedule", [E2D_SCHEDULE_FUSED, E2D_SCHEDULE_PASSES],
+                         ids=["fused", "passes"])
+def test_fp64_prim_quotient_fallback(gpu_available, schedule):
+    """The FP64 primitive conversion divides the momenta and the energy by the
+    density with one shared reciprocal, and a wavefront whose quotients leave
+    the range where that equals IEEE division takes the IEEE division.  Here
+    30% of the cells carry momenta of 0, -0, tiny, denormal and near-threshold
+    magnitudes (both sides of 2^-969): every step's state and t still equal the
+    oracle's bit for bit, so both the fast path and the fallback are exact."""
+    p = _params("blast48x72")
+    p.nStepmax = 8
+    ora = O.OracleRun(p)
+    a = ora.get(0)
+    rng = np.random.default_rng(11)
+    vals = np.array([0.0, -0.0, 1e-300, -1e-300, 5e-320, -5e-320, 1e-310, 2.0**-1000,
+                     -(2.0**-969), 2.0**-968, 3e-292])
+    mask = rng.random(a.shape[:2]) < 0.3
+    for v in (IU, IV):
+        a[..., v][mask] = vals[rng.integers(0, len(vals), size=int(mask.sum()))]
+    for which in (0, 1):
+        ora.set(which, a)
+    run = HydroRun(HydroParams(p.copy()), schedule=schedule)
+    run.set_state(E2D_U, a)
+    run.set_state(E2D_U2, a)
+    ora.run()
+    run.run()
+    _compare(run, ora, f"tiny momenta / {schedule}")
 
 
 def test_autotune_timing_matches_time_loop(gpu_available):
