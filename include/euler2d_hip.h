@@ -251,7 +251,7 @@ int e2d_synchronize(e2d_run *run);
 /* Device-side timing of `nsteps` loop iterations (e2d_run_loop semantics) with
  * HIP events on the handle's stream; returns the elapsed milliseconds and the
  * number of steps actually executed.  Used by benchmarks.  The loop's step
- * launches are replayed from a captured graph of 30 launches where the budget
+ * launches are replayed from captured graphs of 240 or 30 launches where the budget
  * allows (single launches otherwise); a graph missing for the current tile is captured and
  * instantiated (a few milliseconds of host work) before the start event is
  * recorded, so it never falls inside the timed region.  The loop start is one
