@@ -129,6 +129,16 @@ struct P64 {
   static __device__ __forceinline__ void prim(const KParams &kp, const Cell &c, Q4T<T> &q) {
     prim_no_c(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], q);
   }
+  // The march's primitive conversion keeps the reciprocal of the density for
+  // the trace's source terms and checks both with one wave-level test.
+  static constexpr bool kSharedRecip = true;
+  static __device__ __forceinline__ bool prim_shared(const KParams &kp, const Cell &c, Q4T<T> &q,
+                                                     double &r, bool &rok) {
+    return prim_fp64_shared(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], q, r, rok);
+  }
+  static __device__ __forceinline__ void prim_ieee(const KParams &kp, const Cell &c, Q4T<T> &q) {
+    prim_no_c<double, false>(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], q);
+  }
   // internal energy below kin_frac of the total energy (diagnostic only)
   static __device__ __forceinline__ bool kinetic_cell(const KParams &kp, const Cell &c) {
     Q4 q;
@@ -163,6 +173,7 @@ struct P64 {
 struct PSplit {
   using T = float;
   static constexpr unsigned kElem = sizeof(float2);
+  static constexpr bool kSharedRecip = false;
   struct Cell {
     float hi[4], lo[4];
   };
@@ -270,6 +281,7 @@ struct PSplit {
 struct P32 {
   using T = float;
   static constexpr unsigned kElem = sizeof(float);
+  static constexpr bool kSharedRecip = false;
   struct Cell {
     float u[4];
   };
@@ -923,6 +935,10 @@ struct MarchState {
   typename P::Cell un, un2, ua, ub;  // rows r+1 (loaded), r+2 (distance 2), r-1, r-2
   Q4T<typename P::T> qm2, qm1, fymax_c, fx_c, fy_c;
   typename P::T wmax;
+  // FP64: reciprocal of row r-1's density and whether quotients formed from it
+  // can equal the IEEE ones (its density and reciprocal normal)
+  double rm1;
+  bool rokm1;
 };
 
 // One iteration over loaded row r.  STAGES is a compile-time set, so the
@@ -959,7 +975,16 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     }
   }
   Q4T<T> q0;
-  P::prim(kp, uc, q0);
+  [[maybe_unused]] double r0 = 0.0;
+  [[maybe_unused]] bool rok0 = false, qok = true;
+  if constexpr (P::kSharedRecip) {
+    qok = P::prim_shared(kp, uc, q0, r0, rok0);
+    if constexpr ((STAGES & kSlopes) == 0) {
+      if (!wave_all(qok)) P::prim_ieee(kp, uc, q0);  // uniform
+    }
+  } else {
+    P::prim(kp, uc, q0);
+  }
 
   Q4T<T> fya = {}, fxa = {}, fymax = {};
   Cell unew;
@@ -973,7 +998,23 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
         dqY.v[v] = limited_slope(s.slope_type, m.qm1.v[v], q0.v[v], m.qm2.v[v]);
       }
     }
-    const SourcesT<T> src = trace_sources(s, m.qm1, dqX, dqY);
+    SourcesT<T> src;
+    if constexpr (P::kSharedRecip) {
+      // one check for the quotients of row r's conversion and row a's source
+      // terms; a wavefront with any lane outside the range redoes both, and
+      // the y slopes that read row r, with the IEEE divisions
+      src = trace_sources_shared(s, m.qm1, dqX, dqY, m.rm1, m.rokm1, qok);
+      if (!wave_all(qok)) {  // uniform
+        P::prim_ieee(kp, uc, q0);
+        if (s.slope_type != T(0.0)) {
+          for (int v = 0; v < 4; v++)
+            dqY.v[v] = limited_slope(s.slope_type, m.qm1.v[v], q0.v[v], m.qm2.v[v]);
+        }
+        src = trace_sources(s, m.qm1, dqX, dqY);
+      }
+    } else {
+      src = trace_sources(s, m.qm1, dqX, dqY);
+    }
     Q4T<T> fymin;
     trace_face(s, m.qm1, dqY, src, dtdy, false, fymin);
     trace_face(s, m.qm1, dqY, src, dtdy, true, fymax);
@@ -1028,6 +1069,10 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
   if constexpr ((STAGES & kSlopes) != 0) m.fymax_c = fymax;
   m.qm2 = m.qm1;
   m.qm1 = q0;
+  if constexpr (P::kSharedRecip) {
+    m.rm1 = r0;
+    m.rokm1 = rok0;
+  }
   m.ub = m.ua;
   m.ua = uc;
 }
@@ -1106,6 +1151,8 @@ __device__ __forceinline__ typename P::T march(const KParams &kp, const Grid &in
   m.ua = m.un;
   m.ub = m.un;
   m.qm2 = m.qm1 = m.fymax_c = m.fx_c = m.fy_c = Q4T<T>{};
+  m.rm1 = 0.0;
+  m.rokm1 = false;
   m.wmax = T(0.0);
 
   const int r0 = j_base - 2;

@@ -141,15 +141,43 @@ __device__ __forceinline__ bool wave_all(bool ok) {
   return __builtin_amdgcn_ballot_w64(!ok) == 0;
 }
 
-// computePrimitives for FP64 with the reciprocal of d kept: r and whether
-// quotients by d formed from it are the IEEE ones (d and r normal)
-__device__ __forceinline__ void prim_fp64(const SchemeT<double> &s, double u0, double u1,
-                                          double u2, double u3, Q4T<double> &q, double &r,
-                                          bool &rok) {
+// the rest of computePrimitives once the three quotients are formed
+__device__ __forceinline__ void prim_finish(const SchemeT<double> &s, double d, double ux,
+                                            double uy, double e1, Q4T<double> &q) {
+  const double eken = 0.5 * (uy * uy + ux * ux);
+  const double e = e1 - eken;
+  const double p = fmax((s.gamma0 - 1.0) * d * e, d * s.smallp);
+  q.v[ID] = d;
+  q.v[IP] = p;
+  q.v[IU] = ux;
+  q.v[IV] = uy;
+}
+
+// computePrimitives for FP64 with the three quotients formed from one
+// reciprocal and no fallback.  Returns whether this lane's quotients are the
+// IEEE ones; r is the reciprocal and rok whether other quotients by d formed
+// from it can be (d and r normal).
+__device__ __forceinline__ bool prim_fp64_shared(const SchemeT<double> &s, double u0, double u1,
+                                                 double u2, double u3, Q4T<double> &q, double &r,
+                                                 bool &rok) {
   const double d = fmax(u0, s.smallr);
   r = recip_newton(d);
   rok = __builtin_amdgcn_class(d, kClassNormal) && __builtin_amdgcn_class(r, kClassNormal);
   bool ok = rok;
+  const double ux = quot_shared(u2, d, r, ok);
+  const double uy = quot_shared(u3, d, r, ok);
+  const double e1 = quot_shared(u1, d, r, ok);
+  prim_finish(s, d, ux, uy, e1, q);
+  return ok;
+}
+
+// computePrimitives for FP64: the shared reciprocal, and the IEEE divisions
+// for a wavefront with any lane outside its range
+__device__ __forceinline__ void prim_fp64(const SchemeT<double> &s, double u0, double u1,
+                                          double u2, double u3, Q4T<double> &q) {
+  const double d = fmax(u0, s.smallr);
+  const double r = recip_newton(d);
+  bool ok = __builtin_amdgcn_class(d, kClassNormal) && __builtin_amdgcn_class(r, kClassNormal);
   double ux = quot_shared(u2, d, r, ok);
   double uy = quot_shared(u3, d, r, ok);
   double e1 = quot_shared(u1, d, r, ok);
@@ -158,13 +186,7 @@ __device__ __forceinline__ void prim_fp64(const SchemeT<double> &s, double u0, d
     uy = u3 / d;
     e1 = u1 / d;
   }
-  const double eken = 0.5 * (uy * uy + ux * ux);
-  const double e = e1 - eken;
-  const double p = fmax((s.gamma0 - 1.0) * d * e, d * s.smallp);
-  q.v[ID] = d;
-  q.v[IP] = p;
-  q.v[IU] = ux;
-  q.v[IV] = uy;
+  prim_finish(s, d, ux, uy, e1, q);
 }
 
 // computePrimitives, HydroBaseFunctor.h:76-102 (the sound speed is returned
@@ -176,9 +198,7 @@ __device__ __forceinline__ void prim_no_c(const SchemeT<T> &s, T u0, T u1, T u2,
                                           Q4T<T> &q) {
   using O = Ops<T>;
   if constexpr (SHARED && sizeof(T) == sizeof(double)) {
-    double r;
-    bool rok;
-    prim_fp64(s, u0, u1, u2, u3, q, r, rok);
+    prim_fp64(s, u0, u1, u2, u3, q);
   } else {
     const T d = O::mx(u0, s.smallr);
     const T ux = O::div(u2, d);
@@ -279,6 +299,28 @@ __device__ __forceinline__ SourcesT<T> trace_sources(const SchemeT<T> &s, const 
   o.sp0 = O::mad(-(divu * s.gamma0), p, O::mad(-v, dqY.v[IP], -u * dqX.v[IP]));
   o.su0 = O::mad(-v, dqY.v[IU], -u * dqX.v[IU]) - O::div((dqX.v[IP]), r);
   o.sv0 = O::mad(-v, dqY.v[IV], -u * dqX.v[IV]) - O::div((dqY.v[IP]), r);
+  return o;
+}
+
+// trace_sources for FP64 with the two pressure-slope quotients formed from the
+// reciprocal rinv of the density q.v[ID] that the primitive conversion made
+// (rok: d and rinv normal), with no fallback; ok is cleared in a lane whose
+// quotients may differ from the IEEE ones.
+__device__ __forceinline__ Sources trace_sources_shared(const Scheme &s, const Q4 &q,
+                                                        const Q4 &dqX, const Q4 &dqY, double rinv,
+                                                        bool rok, bool &ok) {
+  using O = Ops<double>;
+  const double r = q.v[ID], p = q.v[IP], u = q.v[IU], v = q.v[IV];
+  Sources o;
+  const double divu = dqX.v[IU] + dqY.v[IV];
+  o.sr0 = O::mad(-divu, r, O::mad(-v, dqY.v[ID], -u * dqX.v[ID]));
+  o.sp0 = O::mad(-(divu * s.gamma0), p, O::mad(-v, dqY.v[IP], -u * dqX.v[IP]));
+  bool qok = rok;
+  const double qx = quot_shared(dqX.v[IP], r, rinv, qok);
+  const double qy = quot_shared(dqY.v[IP], r, rinv, qok);
+  ok = ok && qok;
+  o.su0 = O::mad(-v, dqY.v[IU], -u * dqX.v[IU]) - qx;
+  o.sv0 = O::mad(-v, dqY.v[IV], -u * dqX.v[IV]) - qy;
   return o;
 }
 

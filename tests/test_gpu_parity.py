@@ -307,6 +307,40 @@ def test_fp64_prim_quotient_fallback(gpu_available, schedule):
     _compare(run, ora, f"tiny momenta / {schedule}")
 
 
+@pytest.mark.parametrize("schedule", [E2D_SCHEDULE_FUSED, E2D_SCHEDULE_PASSES],
+                         ids=["fused", "passes"])
+def test_fp64_source_quotient_fallback(gpu_available, schedule):
+    """The fused step's trace divides the two pressure slopes by the density
+    with the reciprocal kept from the primitive conversion, and one wave-level
+    check per row covers those quotients and the conversion's.  Every third
+    row holds, over 40 columns at rest, energies of 1e-280 down to 5e-320
+    (both sides of 2^-969): the pressure slopes along x there are below the
+    range of the shared form, while the rows above and below convert exactly,
+    so the check fails on the source terms alone.  smallc = 1e-200 takes the
+    pressure floor to zero (and the wave speeds to the full square root).  Every
+    step's state and t equal the oracle's bit for bit."""
+    p = O.default_params(**dict(CASES["blast48x72"], smallc=1e-200, nStepmax=8))
+    assert p.smallp == 0.0
+    ora = O.OracleRun(p)
+    a = ora.get(0)
+    rng = np.random.default_rng(5)
+    vals = np.array([1e-280, 3e-292, 2.0**-960, 2.0**-968, 2.0**-970, 2.0**-1000, 1e-300,
+                     1e-310, 5e-320])
+    for j in range(2, a.shape[1] - 2, 3):
+        a[5:45, j, 1] = vals[rng.integers(0, len(vals), size=40)]
+        a[5:45, j, IU] = 0.0
+        a[5:45, j, IV] = 0.0
+    for which in (0, 1):
+        ora.set(which, a)
+    run = HydroRun(HydroParams(p.copy()), schedule=schedule)
+    run.set_state(E2D_U, a)
+    run.set_state(E2D_U2, a)
+    ora.run()
+    run.run()
+    assert np.isfinite(ora.current()).all()
+    _compare(run, ora, f"tiny pressure slopes / {schedule}")
+
+
 def test_autotune_timing_matches_time_loop(gpu_available):
     """A candidate's ms per step from the autotuner (60-step runs, whose loop
     graphs are captured before the timed region) agrees with time_loop's on the
