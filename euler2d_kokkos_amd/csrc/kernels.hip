@@ -339,12 +339,17 @@ struct P32 {
   }
 };
 
-// CFL rate of one cell (ComputeDtFunctor body)
-template <class P>
+// CFL rate of one cell (ComputeDtFunctor body); FS: the FP64 sound speed
+// takes the short square root (kernels built for the host's floor_sqrt)
+template <class P, bool FS = false>
 __device__ __forceinline__ typename P::T cell_rate(const KParams &kp, const typename P::Cell &c) {
-  Q4T<typename P::T> q;
-  P::prim(kp, c, q);
-  return cfl_rate_q(P::scheme(kp), q, P::rate(kp));
+  if constexpr (P::kSharedRecip) {
+    return cfl_rate_fp64<FS>(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], kp.rd);
+  } else {
+    Q4T<typename P::T> q;
+    P::prim(kp, c, q);
+    return cfl_rate_q(P::scheme(kp), q, P::rate(kp));
+  }
 }
 
 __device__ __forceinline__ double &at(const Grid &g, int v, int i, int j) {
@@ -1059,7 +1064,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     if (out_col && (!ROWS || b <= kp.ny + 1)) {
       P::template store_rc<O32>(out, b, pin_offset((unsigned)col * P::kElem), unew);
       if constexpr (RATE)  // next-step CFL rate of the new state
-        m.wmax = Ops<T>::mx(m.wmax, cell_rate<P>(kp, unew));
+        m.wmax = Ops<T>::mx(m.wmax, cell_rate<P, (RS & kRiemannFloorSqrt) != 0>(kp, unew));
     }
   }
   if constexpr ((STAGES & kYFace) != 0) {
@@ -1776,7 +1781,12 @@ KParams make_kparams(const e2d_params &p, int precision) {
   k.s.slope_type = p.slope_type;
   k.s.entho = 1.0 / (p.gamma0 - 1.0);
   k.s.smallc2 = p.smallc * p.smallc;
-  k.s.floor_sqrt = k.s.smallc2 >= std::ldexp(1.0, -759) ? 1 : 0;
+  // the short square root for the wave speeds (arguments >= smallc^2) and
+  // the CFL rate (arguments >= about smallc^2 while d smallp is normal)
+  k.s.floor_sqrt =
+      k.s.smallc2 >= std::ldexp(1.0, -759) && p.smallr * p.smallp >= std::ldexp(1.0, -1000)
+          ? 1
+          : 0;
   k.s.riemann = p.honor_riemann ? p.riemannSolverType : 2;
   k.sf.gamma0 = (float)k.s.gamma0;
   k.sf.gamma6 = (float)k.s.gamma6;

@@ -251,6 +251,50 @@ __device__ __forceinline__ T cfl_rate_q(const SchemeT<T> &s, const Q4T<T> &q, co
   return rx + ry;
 }
 
+// The rate's square root of gamma p / d.  p >= d smallp, and with smallc^2 >=
+// 2^-759 and smallr smallp >= 2^-1000 (the host's floor_sqrt condition, which
+// keeps d smallp normal for every d >= smallr) the argument is at least
+// smallc^2 (1 - 5u) > 2^-767, where the short sequence gives the IEEE bits;
+// +inf keeps sqrt's result, and NaN stays NaN.
+template <bool FS>
+__device__ __forceinline__ double rate_sqrt(double x) {
+  if constexpr (FS) {
+    constexpr int kClassPosInf = 1 << 9;
+    return __builtin_amdgcn_class(x, kClassPosInf) ? x : sqrt_core(x);
+  } else {
+    return sqrt(x);
+  }
+}
+
+// cfl_rate_q for FP64 from the conserved state: the quotient gamma p / d
+// shares the primitive conversion's reciprocal and its one wave-level check
+template <bool FS>
+__device__ __forceinline__ double cfl_rate_fp64(const Scheme &s, double u0, double u1, double u2,
+                                                double u3, const RateT<double> &rt) {
+  Q4 q;
+  double r;
+  bool rok;
+  bool ok = prim_fp64_shared(s, u0, u1, u2, u3, q, r, rok);
+  double x = quot_shared(s.gamma0 * q.v[IP], q.v[ID], r, rok);
+  if (!wave_all(ok && rok)) {  // uniform
+    prim_no_c<double, false>(s, u0, u1, u2, u3, q);
+    x = (s.gamma0 * q.v[IP]) / q.v[ID];
+  }
+  const double c = rate_sqrt<FS>(x);
+  const double vx = c + fabs(q.v[IU]);
+  const double vy = c + fabs(q.v[IV]);
+  double rx, ry;
+  if (rt.pow2x)  // uniform
+    rx = vx * rt.idx;
+  else
+    rx = vx / rt.dx;
+  if (rt.pow2y)
+    ry = vy * rt.idy;
+  else
+    ry = vy / rt.dy;
+  return rx + ry;
+}
+
 __device__ __forceinline__ double cfl_rate(const Scheme &s, double u0, double u1, double u2,
                                            double u3, double dx, double dy) {
   Q4 q;
