@@ -114,9 +114,10 @@ using Q4 = Q4T<double>;
 // takes the IEEE division for all of its lanes.  A zero quotient takes the
 // sign of m, as div_fixup gives it.  Same bits; 27 instead of 33 VALU
 // instructions for the three quotients of prim.  Each fallback is a branch
-// that splits the march's steady row; sharing the reciprocal also with the
-// sound speed's quotient of the CFL rate (a third branch per row) measured
-// slower on the 1024x1536 step, so that quotient stays an IEEE division.
+// that splits the march's steady row, and a third branch per row measured
+// slower than no sharing at all; so quotients by one density that a row forms
+// later (the trace's source terms, the CFL rate's gamma p / d) join the check
+// of that density's conversion instead of adding one.
 
 __device__ __forceinline__ double recip_newton(double d) {
   const double r0 = __builtin_amdgcn_rcp(d);
