@@ -997,11 +997,10 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     const Q4T<T> qpx = shfl_dn1(m.qm1);
     const Q4T<T> qmx = shfl_up1(m.qm1);
     Q4T<T> dqX = {}, dqY = {};
-    if (s.slope_type != T(0.0)) {  // uniform: one scalar branch per row
-      for (int v = 0; v < 4; v++) {
-        dqX.v[v] = limited_slope(s.slope_type, m.qm1.v[v], qpx.v[v], qmx.v[v]);
-        dqY.v[v] = limited_slope(s.slope_type, m.qm1.v[v], q0.v[v], m.qm2.v[v]);
-      }
+    if constexpr ((RS & kSlopeMinmod) != 0) {  // slope_type == 1: no branch, no products by one
+      slopes_xy<true>(s.slope_type, m.qm1, qpx, qmx, q0, m.qm2, dqX, dqY);
+    } else if (s.slope_type != T(0.0)) {  // uniform: one scalar branch per row
+      slopes_xy<false>(s.slope_type, m.qm1, qpx, qmx, q0, m.qm2, dqX, dqY);
     }
     SourcesT<T> src;
     if constexpr (P::kSharedRecip) {
@@ -1011,10 +1010,10 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
       src = trace_sources_shared(s, m.qm1, dqX, dqY, m.rm1, m.rokm1, qok);
       if (!wave_all(qok)) {  // uniform
         P::prim_ieee(kp, uc, q0);
-        if (s.slope_type != T(0.0)) {
-          for (int v = 0; v < 4; v++)
-            dqY.v[v] = limited_slope(s.slope_type, m.qm1.v[v], q0.v[v], m.qm2.v[v]);
-        }
+        if constexpr ((RS & kSlopeMinmod) != 0)
+          slopes_dir<true>(s.slope_type, m.qm1, q0, m.qm2, dqY);
+        else if (s.slope_type != T(0.0))
+          slopes_dir<false>(s.slope_type, m.qm1, q0, m.qm2, dqY);
         src = trace_sources(s, m.qm1, dqX, dqY);
       }
     } else {
@@ -1736,12 +1735,13 @@ void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
     else
       hipLaunchKernelGGL((step_fused_kernel<RS, P, true, true, 1>), dim3(blocks), dim3(kBlock), 0,
                          st, kp, in, out, dt, ctl, ph, fm);
-  } else if (ctl) {
-    hipLaunchKernelGGL((step_fused_kernel<RS, P, true, false, 1>), dim3(blocks), dim3(kBlock), 0,
-                       st, kp, in, out, dt, ctl, ph, fm);
-  } else {
-    hipLaunchKernelGGL((step_fused_kernel<RS, P, false, false, 1>), dim3(blocks), dim3(kBlock), 0,
-                       st, kp, in, out, dt, ctl, ph, fm);
+  } else if constexpr ((RS & kSlopeMinmod) == 0) {  // the minmod kinds have the branch above only
+    if (ctl)
+      hipLaunchKernelGGL((step_fused_kernel<RS, P, true, false, 1>), dim3(blocks), dim3(kBlock),
+                         0, st, kp, in, out, dt, ctl, ph, fm);
+    else
+      hipLaunchKernelGGL((step_fused_kernel<RS, P, false, false, 1>), dim3(blocks), dim3(kBlock),
+                         0, st, kp, in, out, dt, ctl, ph, fm);
   }
 }
 
@@ -1750,6 +1750,16 @@ void launch_fused_p(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
                     int seg_rows, hipStream_t st) {
   if constexpr (std::is_same<P, P64>::value) {
     if (kp.s.floor_sqrt && kp.s.riemann != 0) {  // HLL / HLLC with the short square root
+      // minmod slopes built in: run-loop launches with 32-bit row offsets only
+      if (kp.s.slope_type == 1.0 && ctl && off32_ok(kp, in) && off32_ok(kp, out)) {
+        if (kp.s.riemann == 2)
+          launch_fused_t<2 | kRiemannFloorSqrt | kSlopeMinmod, P>(kp, in, out, dt, ctl, ph,
+                                                                 seg_rows, st);
+        else
+          launch_fused_t<1 | kRiemannFloorSqrt | kSlopeMinmod, P>(kp, in, out, dt, ctl, ph,
+                                                                 seg_rows, st);
+        return;
+      }
       if (kp.s.riemann == 2)
         launch_fused_t<2 | kRiemannFloorSqrt, P>(kp, in, out, dt, ctl, ph, seg_rows, st);
       else

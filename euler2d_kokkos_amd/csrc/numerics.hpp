@@ -83,7 +83,7 @@ struct SchemeT {
   T gamma0, gamma6, smallr, smallc, smallp, smallpp, slope_type, entho;
   T smallc2;    // smallc * smallc
   int riemann;  // 0 approx, 1 hll, 2 hllc (already resolved from honor_riemann)
-  int floor_sqrt;  // smallc2 >= 2^-759: floored sound-speed roots take the short sqrt
+  int floor_sqrt;  // smallc2 >= 2^-759 and smallr*smallp >= 2^-1000: sound speeds take the short sqrt
 };
 using Scheme = SchemeT<double>;
 
@@ -303,17 +303,40 @@ __device__ __forceinline__ double cfl_rate(const Scheme &s, double u0, double u1
   return cfl_rate_q(s, q, dx, dy);
 }
 
-// slope_unsplit_hydro_2d_scalar, HydroBaseFunctor.h:419-455 (one direction)
-template <class T>
+// slope_unsplit_hydro_2d_scalar, HydroBaseFunctor.h:419-455 (one direction).
+// MINMOD: st == 1, whose products with the one-sided differences are exact
+// identities (every value, zeros, infinities and NaN included) and are left out.
+template <bool MINMOD = false, class T>
 __device__ __forceinline__ T limited_slope(T st, T q, T qplus, T qminus) {
   using O = Ops<T>;
-  const T dlft = st * (q - qminus);
-  const T drgt = st * (qplus - q);
+  const T dlft = MINMOD ? q - qminus : st * (q - qminus);
+  const T drgt = MINMOD ? qplus - q : st * (qplus - q);
   const T dcen = T(0.5) * (qplus - qminus);
   const T dsgn = (dcen >= T(0.0)) ? T(1.0) : T(-1.0);
   const T slop = O::mn(O::ab(dlft), O::ab(drgt));
   const T dlim = ((dlft * drgt) <= T(0.0)) ? T(0.0) : slop;
   return dsgn * O::mn(dlim, O::ab(dcen));
+}
+
+// The limited slopes of the four variables of a cell in one direction, from
+// its neighbours qp and qm, for slope_type == 1 (MINMOD) or any other non-zero
+// slope_type
+template <bool MINMOD, class T>
+__device__ __forceinline__ void slopes_dir(T st, const Q4T<T> &q, const Q4T<T> &qp,
+                                           const Q4T<T> &qm, Q4T<T> &dq) {
+  for (int v = 0; v < 4; v++) dq.v[v] = limited_slope<MINMOD>(st, q.v[v], qp.v[v], qm.v[v]);
+}
+
+// Both directions, variable by variable (the order the split step's register
+// allocation was tuned with)
+template <bool MINMOD, class T>
+__device__ __forceinline__ void slopes_xy(T st, const Q4T<T> &q, const Q4T<T> &qpx,
+                                          const Q4T<T> &qmx, const Q4T<T> &qpy,
+                                          const Q4T<T> &qmy, Q4T<T> &dqX, Q4T<T> &dqY) {
+  for (int v = 0; v < 4; v++) {
+    dqX.v[v] = limited_slope<MINMOD>(st, q.v[v], qpx.v[v], qmx.v[v]);
+    dqY.v[v] = limited_slope<MINMOD>(st, q.v[v], qpy.v[v], qmy.v[v]);
+  }
 }
 
 // slope_unsplit_hydro_2d, HydroBaseFunctor.h:473-516 for one variable: zero
@@ -576,6 +599,9 @@ __device__ __forceinline__ void approx(const SchemeT<T> &s, const Q4T<T> &qlf, c
 // RS: 0 approx, 1 HLL, 2 HLLC; + kRiemannFloorSqrt: the HLL/HLLC wave speeds
 // use the short square root (FP64, arguments floored at smallc^2 >= 2^-759)
 constexpr int kRiemannFloorSqrt = 8;
+// + kSlopeMinmod: the march's slopes are built for slope_type == 1 (the host
+// selects such a kernel only then)
+constexpr int kSlopeMinmod = 16;
 
 template <int RS, class T>
 __device__ __forceinline__ void riemann(const SchemeT<T> &s, const Q4T<T> &ql, const Q4T<T> &qr,
