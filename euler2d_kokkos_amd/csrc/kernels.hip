@@ -340,11 +340,12 @@ struct P32 {
 };
 
 // CFL rate of one cell (ComputeDtFunctor body); FS: the FP64 sound speed
-// takes the short square root (kernels built for the host's floor_sqrt)
-template <class P, bool FS = false>
+// takes the short square root (kernels built for the host's floor_sqrt);
+// POW2: the cell sizes are powers of two
+template <class P, bool FS = false, bool POW2 = false>
 __device__ __forceinline__ typename P::T cell_rate(const KParams &kp, const typename P::Cell &c) {
   if constexpr (P::kSharedRecip) {
-    return cfl_rate_fp64<FS>(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], kp.rd);
+    return cfl_rate_fp64<FS, POW2>(kp.s, c.u[0], c.u[1], c.u[2], c.u[3], kp.rd);
   } else {
     Q4T<typename P::T> q;
     P::prim(kp, c, q);
@@ -997,7 +998,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     const Q4T<T> qpx = shfl_dn1(m.qm1);
     const Q4T<T> qmx = shfl_up1(m.qm1);
     Q4T<T> dqX = {}, dqY = {};
-    if constexpr ((RS & kSlopeMinmod) != 0) {  // slope_type == 1: no branch, no products by one
+    if constexpr ((RS & kMinmodPow2) != 0) {  // slope_type == 1: no branch, no products by one
       slopes_xy<true>(s.slope_type, m.qm1, qpx, qmx, q0, m.qm2, dqX, dqY);
     } else if (s.slope_type != T(0.0)) {  // uniform: one scalar branch per row
       slopes_xy<false>(s.slope_type, m.qm1, qpx, qmx, q0, m.qm2, dqX, dqY);
@@ -1010,7 +1011,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
       src = trace_sources_shared(s, m.qm1, dqX, dqY, m.rm1, m.rokm1, qok);
       if (!wave_all(qok)) {  // uniform
         P::prim_ieee(kp, uc, q0);
-        if constexpr ((RS & kSlopeMinmod) != 0)
+        if constexpr ((RS & kMinmodPow2) != 0)
           slopes_dir<true>(s.slope_type, m.qm1, q0, m.qm2, dqY);
         else if (s.slope_type != T(0.0))
           slopes_dir<false>(s.slope_type, m.qm1, q0, m.qm2, dqY);
@@ -1063,7 +1064,8 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     if (out_col && (!ROWS || b <= kp.ny + 1)) {
       P::template store_rc<O32>(out, b, pin_offset((unsigned)col * P::kElem), unew);
       if constexpr (RATE)  // next-step CFL rate of the new state
-        m.wmax = Ops<T>::mx(m.wmax, cell_rate<P, (RS & kRiemannFloorSqrt) != 0>(kp, unew));
+        m.wmax = Ops<T>::mx(m.wmax, cell_rate<P, (RS & kRiemannFloorSqrt) != 0,
+                                              (RS & kMinmodPow2) != 0>(kp, unew));
     }
   }
   if constexpr ((STAGES & kYFace) != 0) {
@@ -1735,7 +1737,7 @@ void launch_fused_t(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
     else
       hipLaunchKernelGGL((step_fused_kernel<RS, P, true, true, 1>), dim3(blocks), dim3(kBlock), 0,
                          st, kp, in, out, dt, ctl, ph, fm);
-  } else if constexpr ((RS & kSlopeMinmod) == 0) {  // the minmod kinds have the branch above only
+  } else if constexpr ((RS & kMinmodPow2) == 0) {  // the minmod kinds have the branch above only
     if (ctl)
       hipLaunchKernelGGL((step_fused_kernel<RS, P, true, false, 1>), dim3(blocks), dim3(kBlock),
                          0, st, kp, in, out, dt, ctl, ph, fm);
@@ -1750,13 +1752,15 @@ void launch_fused_p(const KParams &kp, Grid in, Grid out, double dt, StepCtl *ct
                     int seg_rows, hipStream_t st) {
   if constexpr (std::is_same<P, P64>::value) {
     if (kp.s.floor_sqrt && kp.s.riemann != 0) {  // HLL / HLLC with the short square root
-      // minmod slopes built in: run-loop launches with 32-bit row offsets only
-      if (kp.s.slope_type == 1.0 && ctl && off32_ok(kp, in) && off32_ok(kp, out)) {
+      // minmod slopes and power-of-two cell sizes built in: run-loop launches
+      // with 32-bit row offsets only
+      if (kp.s.slope_type == 1.0 && kp.rd.pow2x && kp.rd.pow2y && ctl && off32_ok(kp, in) &&
+          off32_ok(kp, out)) {
         if (kp.s.riemann == 2)
-          launch_fused_t<2 | kRiemannFloorSqrt | kSlopeMinmod, P>(kp, in, out, dt, ctl, ph,
+          launch_fused_t<2 | kRiemannFloorSqrt | kMinmodPow2, P>(kp, in, out, dt, ctl, ph,
                                                                  seg_rows, st);
         else
-          launch_fused_t<1 | kRiemannFloorSqrt | kSlopeMinmod, P>(kp, in, out, dt, ctl, ph,
+          launch_fused_t<1 | kRiemannFloorSqrt | kMinmodPow2, P>(kp, in, out, dt, ctl, ph,
                                                                  seg_rows, st);
         return;
       }

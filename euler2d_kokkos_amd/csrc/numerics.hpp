@@ -261,15 +261,20 @@ template <bool FS>
 __device__ __forceinline__ double rate_sqrt(double x) {
   if constexpr (FS) {
     constexpr int kClassPosInf = 1 << 9;
-    return __builtin_amdgcn_class(x, kClassPosInf) ? x : sqrt_core(x);
+    double r = sqrt_core(x);
+    // formed in every lane and then selected: a branch around the root would
+    // split the march's row for the lanes holding +inf
+    asm volatile("" : "+v"(r));
+    return __builtin_amdgcn_class(x, kClassPosInf) ? x : r;
   } else {
     return sqrt(x);
   }
 }
 
 // cfl_rate_q for FP64 from the conserved state: the quotient gamma p / d
-// shares the primitive conversion's reciprocal and its one wave-level check
-template <bool FS>
+// shares the primitive conversion's reciprocal and its one wave-level check.
+// POW2: both cell sizes are known to be powers of two (no test per cell).
+template <bool FS, bool POW2 = false>
 __device__ __forceinline__ double cfl_rate_fp64(const Scheme &s, double u0, double u1, double u2,
                                                 double u3, const RateT<double> &rt) {
   Q4 q;
@@ -284,6 +289,7 @@ __device__ __forceinline__ double cfl_rate_fp64(const Scheme &s, double u0, doub
   const double c = rate_sqrt<FS>(x);
   const double vx = c + fabs(q.v[IU]);
   const double vy = c + fabs(q.v[IV]);
+  if constexpr (POW2) return vx * rt.idx + vy * rt.idy;
   double rx, ry;
   if (rt.pow2x)  // uniform
     rx = vx * rt.idx;
@@ -599,9 +605,10 @@ __device__ __forceinline__ void approx(const SchemeT<T> &s, const Q4T<T> &qlf, c
 // RS: 0 approx, 1 HLL, 2 HLLC; + kRiemannFloorSqrt: the HLL/HLLC wave speeds
 // use the short square root (FP64, arguments floored at smallc^2 >= 2^-759)
 constexpr int kRiemannFloorSqrt = 8;
-// + kSlopeMinmod: the march's slopes are built for slope_type == 1 (the host
-// selects such a kernel only then)
-constexpr int kSlopeMinmod = 16;
+// + kMinmodPow2: the march is built for slope_type == 1 (its slopes) and for
+// cell sizes that are powers of two (its CFL rate); the host selects such a
+// kernel only then
+constexpr int kMinmodPow2 = 16;
 
 template <int RS, class T>
 __device__ __forceinline__ void riemann(const SchemeT<T> &s, const Q4T<T> &ql, const Q4T<T> &qr,
