@@ -1111,6 +1111,7 @@ struct e2d_batch {
   int graph_steps = 0;
   int graph_tile = 0;
   int floor_sqrt = 1;  // every instance allows the short wave-speed square root
+  int minmod = 1;      // every instance has slope_type 1
   // persistent launch (E2D_BATCH_LAUNCH_PERSISTENT)
   int launch = E2D_BATCH_LAUNCH_PERSISTENT;
   double queue_imbalance = 1.0;          // cells of the fullest XCD queue / the mean
@@ -1240,7 +1241,7 @@ int batch_launch(e2d_batch *b) {
   k0.s.floor_sqrt = b->floor_sqrt;
   for (const e2d_run *r : b->runs) k0.wide_addr |= r->kp.wide_addr;
   HIP_TRY(e2d::launch_step_batch(k0, b->insts, (int)b->runs.size(), b->ctls, b->phase,
-                                 b->nwaves, b->seg_rows, b->stream));
+                                 b->nwaves, b->seg_rows, b->stream, b->minmod != 0));
   b->phase = b->phase == 2 ? 0 : b->phase + 1;
   return E2D_OK;
 }
@@ -1316,7 +1317,7 @@ int batch_run_persistent(e2d_batch *b, int max_steps, double *elapsed_ms, int *a
     HIP_TRY(e2d::launch_step_batch_persistent(
         k0, b->insts, (int)b->runs.size(), b->ctls, b->phase, steps, b->queue, b->counters,
         done, b->seg_rows, b->spin_limit ? b->spin_limit : kDefaultSpinLimit, b->ncu,
-        b->stream));
+        b->stream, b->minmod != 0));
     b->phase = (b->phase + steps) % 3;
     left -= steps;
     e2d::BatchCounters c{};
@@ -1397,6 +1398,7 @@ int e2d_batch_create(const e2d_params *params, int count, int precision, e2d_bat
       }
       b->runs.push_back(r);
       b->floor_sqrt = b->floor_sqrt && r->kp.s.floor_sqrt;
+      b->minmod = b->minmod && r->kp.s.slope_type == 1.0;
       if (r->kp.s.riemann != b->runs[0]->kp.s.riemann) {
         e2d_batch_destroy(b);
         return fail(E2D_ERR_UNSUPPORTED, "all instances of a batch must use the same Riemann solver");

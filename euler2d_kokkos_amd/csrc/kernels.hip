@@ -998,7 +998,8 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
     const Q4T<T> qpx = shfl_dn1(m.qm1);
     const Q4T<T> qmx = shfl_up1(m.qm1);
     Q4T<T> dqX = {}, dqY = {};
-    if constexpr ((RS & kMinmodPow2) != 0) {  // slope_type == 1: no branch, no products by one
+    if constexpr ((RS & (kMinmodPow2 | kSlopeMinmod)) != 0) {  // slope_type == 1: no branch,
+                                                                // no products by one
       slopes_xy<true>(s.slope_type, m.qm1, qpx, qmx, q0, m.qm2, dqX, dqY);
     } else if (s.slope_type != T(0.0)) {  // uniform: one scalar branch per row
       slopes_xy<false>(s.slope_type, m.qm1, qpx, qmx, q0, m.qm2, dqX, dqY);
@@ -1011,7 +1012,7 @@ __device__ __forceinline__ void march_row(const KParams &kp, const Grid &in, con
       src = trace_sources_shared(s, m.qm1, dqX, dqY, m.rm1, m.rokm1, qok);
       if (!wave_all(qok)) {  // uniform
         P::prim_ieee(kp, uc, q0);
-        if constexpr ((RS & kMinmodPow2) != 0)
+        if constexpr ((RS & (kMinmodPow2 | kSlopeMinmod)) != 0)
           slopes_dir<true>(s.slope_type, m.qm1, q0, m.qm2, dqY);
         else if (s.slope_type != T(0.0))
           slopes_dir<false>(s.slope_type, m.qm1, q0, m.qm2, dqY);
@@ -2032,7 +2033,8 @@ hipError_t launch_persistent(K kernel, int ncu, hipStream_t st, const BatchInst 
 hipError_t launch_step_batch_persistent(const KParams &kp0, const BatchInst *insts_dev, int ninst,
                                         StepCtl *ctls_dev, int ph0, int steps, BatchQueue q,
                                         BatchCounters *counters, unsigned *done, int seg_rows,
-                                        unsigned spin_limit, int ncu, hipStream_t st) {
+                                        unsigned spin_limit, int ncu, hipStream_t st,
+                                        bool minmod) {
   (void)ninst;
   if (ph0 < 0 || ph0 > 2 || steps <= 0 || ncu <= 0) return hipErrorInvalidValue;
 #define E2D_PBATCH_ARGS ncu, st, insts_dev, ctls_dev, ph0, steps, q, counters, done, seg_rows, \
@@ -2051,6 +2053,10 @@ hipError_t launch_step_batch_persistent(const KParams &kp0, const BatchInst *ins
     E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 0, P32);
   }
   if (kp0.s.floor_sqrt && kp0.s.riemann != 0) {
+    if (minmod && kp0.s.riemann == 2)
+      E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 2 | kRiemannFloorSqrt | kSlopeMinmod, P64);
+    if (minmod)
+      E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 1 | kRiemannFloorSqrt | kSlopeMinmod, P64);
     if (kp0.s.riemann == 2)
       E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 2 | kRiemannFloorSqrt, P64);
     E2D_PBATCH_LAUNCH(step_batch_persistent_kernel, 1 | kRiemannFloorSqrt, P64);
@@ -2063,7 +2069,8 @@ hipError_t launch_step_batch_persistent(const KParams &kp0, const BatchInst *ins
 }
 
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
-                             StepCtl *ctls_dev, int ph, int nwaves, int seg_rows, hipStream_t st) {
+                             StepCtl *ctls_dev, int ph, int nwaves, int seg_rows, hipStream_t st,
+                             bool minmod) {
   if (ph < 0 || ph > 2) return hipErrorInvalidValue;
   const int blocks = (nwaves + kBlock / kLanes - 1) / (kBlock / kLanes);
 #define E2D_BATCH_LAUNCH(K, RS, P)                                                            \
@@ -2083,6 +2090,11 @@ hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int
     if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(step_batch_kernel, 2, P32);
     else if (kp0.s.riemann == 1) E2D_BATCH_LAUNCH(step_batch_kernel, 1, P32);
     else E2D_BATCH_LAUNCH(step_batch_kernel, 0, P32);
+  } else if (kp0.s.floor_sqrt && kp0.s.riemann != 0 && minmod) {
+    if (kp0.s.riemann == 2)
+      E2D_BATCH_LAUNCH(step_batch_kernel, 2 | kRiemannFloorSqrt | kSlopeMinmod, P64);
+    else
+      E2D_BATCH_LAUNCH(step_batch_kernel, 1 | kRiemannFloorSqrt | kSlopeMinmod, P64);
   } else if (kp0.s.floor_sqrt && kp0.s.riemann != 0) {
     if (kp0.s.riemann == 2) E2D_BATCH_LAUNCH(step_batch_kernel, 2 | kRiemannFloorSqrt, P64);
     else E2D_BATCH_LAUNCH(step_batch_kernel, 1 | kRiemannFloorSqrt, P64);

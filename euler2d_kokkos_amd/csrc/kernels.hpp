@@ -186,9 +186,11 @@ int sums_partials_needed(const KParams &kp);
 int batch_waves(const KParams &kp, int seg_rows, int *nstrips);
 // one step of every active instance; kp0 gives the precision and solver of all
 // (kp0.s.floor_sqrt must hold for every instance; kp0.wide_addr must be set
-// when any instance has a plane above 4 GiB)
+// when any instance has a plane above 4 GiB; minmod: every instance has
+// slope_type 1)
 hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int ninst,
-                             StepCtl *ctls_dev, int ph, int nwaves, int seg_rows, hipStream_t st);
+                             StepCtl *ctls_dev, int ph, int nwaves, int seg_rows, hipStream_t st,
+                             bool minmod = false);
 // All `steps` steps of every instance in one persistent launch: as many blocks
 // of 256 threads as the kernel fits on `ncu` CUs at once.  counters and done
 // (one word every kCounterStride per instance) must be zero; steps times the
@@ -198,7 +200,8 @@ hipError_t launch_step_batch(const KParams &kp0, const BatchInst *insts_dev, int
 hipError_t launch_step_batch_persistent(const KParams &kp0, const BatchInst *insts_dev, int ninst,
                                         StepCtl *ctls_dev, int ph0, int steps, BatchQueue q,
                                         BatchCounters *counters, unsigned *done, int seg_rows,
-                                        unsigned spin_limit, int ncu, hipStream_t st);
+                                        unsigned spin_limit, int ncu, hipStream_t st,
+                                        bool minmod = false);
 // XCC id of the XCD each of `blocks` one-wavefront blocks ran on, into out[block]
 hipError_t launch_xcc_probe(unsigned *out, int blocks, hipStream_t st);
 // per-bin cell counts and density sums of the radial profile (nbins entries each)

@@ -609,6 +609,9 @@ constexpr int kRiemannFloorSqrt = 8;
 // cell sizes that are powers of two (its CFL rate); the host selects such a
 // kernel only then
 constexpr int kMinmodPow2 = 16;
+// + kSlopeMinmod: the march's slopes alone are built for slope_type == 1 (the
+// batch kernels, whose instances' cell sizes may be anything)
+constexpr int kSlopeMinmod = 32;
 
 template <int RS, class T>
 __device__ __forceinline__ void riemann(const SchemeT<T> &s, const Q4T<T> &ql, const Q4T<T> &qr,
