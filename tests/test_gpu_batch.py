@@ -102,10 +102,18 @@ def test_batch_wide_addressing(gpu_available, precision, launch, monkeypatch):
                                       err_msg=f"instance {i}")
 
 
+@pytest.mark.parametrize("slopes", ["minmod", "mixed"])
 @pytest.mark.parametrize("launch", [E2D_BATCH_LAUNCH_STEPS, E2D_BATCH_LAUNCH_PERSISTENT],
                          ids=["steps", "persistent"])
-def test_batch_fp64_equals_oracle(gpu_available, launch):
+def test_batch_fp64_equals_oracle(gpu_available, launch, slopes):
+    """Every instance's state and t equal the oracle's bit for bit.  "minmod":
+    all instances have slope_type 1, so the batch runs its minmod kernels;
+    "mixed": one instance has the MC limiter (slope_type 2), so the batch runs
+    the generic ones."""
     ps = mixed_instances()[:3]
+    if slopes == "mixed":
+        ps[1] = make_params(nx=128, ny=64, problemType=1, cfl=0.7, nStepmax=25, tEnd=1e9,
+                            blast_radius=0.15, blast_pressure_in=50.0, slope_type=2.0)
     batch = HydroBatch(ps, precision=E2D_PRECISION_FP64)
     batch.set_launch(launch)
     batch.run(40)
